@@ -1,0 +1,160 @@
+"""Structured 2D quad mesh, boundary-condition types and affine mesh metrics (host side).
+
+Follows FRForge `src/mesh/Mesh2D.jl`, `src/mesh/BoundaryConditions.jl` and the affine branch
+of `src/mesh/MeshMetrics2D.jl`.  Element index ``e = jy * nx + jx`` (0-based here; the
+reference is 1-based, Mesh2D.jl:100).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .operators import FROperators
+
+_EPS = np.finfo(np.float64).eps
+
+
+class AbstractBC:
+    """Boundary-condition tag (BoundaryConditions.jl:6)."""
+
+
+class PeriodicBC(AbstractBC):
+    """Periodic domain end; both ends of an axis must be periodic (BoundaryConditions.jl:9)."""
+
+    def __repr__(self):
+        return "PeriodicBC()"
+
+
+class TransmissiveBC(AbstractBC):
+    """Zero-order extrapolation: ghost state = interior trace (BoundaryConditions.jl:12)."""
+
+    def __repr__(self):
+        return "TransmissiveBC()"
+
+
+class ReflectingBC(AbstractBC):
+    """Slip wall (BoundaryConditions.jl:27).  Not supported by the device path."""
+
+
+class DirichletBC(AbstractBC):
+    """Prescribed state (BoundaryConditions.jl:22).  Not supported by the device path."""
+
+    def __init__(self, u_func):
+        self.u_func = u_func
+
+
+def _linspace(a: float, b: float, n: int) -> np.ndarray:
+    """Vertices a + k (b - a) / n, k = 0..n; correctly rounded whenever the reference's
+    ``range(a, b; length=n+1)`` is exact (all meshes used here: power-of-two spacings)."""
+    k = np.arange(n + 1, dtype=np.float64)
+    v = a + (k * (b - a)) / float(n)
+    v[0], v[-1] = a, b
+    return v
+
+
+@dataclass
+class Mesh2D:
+    """``nx x ny`` affine quad mesh (Mesh2D.jl:11-91)."""
+
+    x_left: float
+    x_right: float
+    y_bottom: float
+    y_top: float
+    nx: int
+    ny: int
+    left_bc: AbstractBC = field(default_factory=PeriodicBC)
+    right_bc: AbstractBC = field(default_factory=PeriodicBC)
+    bottom_bc: AbstractBC = field(default_factory=PeriodicBC)
+    top_bc: AbstractBC = field(default_factory=PeriodicBC)
+
+    def __post_init__(self):
+        if self.nx < 1 or self.ny < 1:
+            raise ValueError("nx, ny must be >= 1")
+        if not self.x_right > self.x_left:
+            raise ValueError("x_right > x_left required")
+        if not self.y_top > self.y_bottom:
+            raise ValueError("y_top > y_bottom required")
+        if isinstance(self.left_bc, PeriodicBC) != isinstance(self.right_bc, PeriodicBC):
+            raise ValueError("x-periodic requires both left and right PeriodicBC")
+        if isinstance(self.bottom_bc, PeriodicBC) != isinstance(self.top_bc, PeriodicBC):
+            raise ValueError("y-periodic requires both bottom and top PeriodicBC")
+        self.x_vertices = _linspace(float(self.x_left), float(self.x_right), self.nx)
+        self.y_vertices = _linspace(float(self.y_bottom), float(self.y_top), self.ny)
+        self.dx = np.diff(self.x_vertices)
+        self.dy = np.diff(self.y_vertices)
+        self.Jx = self.dx / 2.0
+        self.Jy = self.dy / 2.0
+
+    @property
+    def n_elements(self) -> int:
+        return self.nx * self.ny
+
+    def element_index(self, jx: int, jy: int) -> int:
+        return jy * self.nx + jx
+
+    def element_coords(self, e: int):
+        jy, jx = divmod(e, self.nx)
+        return jx, jy
+
+    def sp_coordinates(self, ops: FROperators):
+        """Physical (x, y) of every solution point, arrays of shape (Np, Np, Nel) in
+        Fortran order (physical_xy, MeshMetrics2D.jl:49-58)."""
+        Np, nx = ops.n_points, self.nx
+        xc = (self.x_vertices[:-1] + self.x_vertices[1:]) / 2.0     # (nx,)
+        yc = (self.y_vertices[:-1] + self.y_vertices[1:]) / 2.0     # (ny,)
+        X1 = xc[None, :] + self.Jx[None, :] * ops.xi[:, None]        # X1[i, jx]
+        Y1 = yc[None, :] + self.Jy[None, :] * ops.xi[:, None]        # Y1[j, jy]
+        e = np.arange(self.n_elements)
+        jx, jy = e % nx, e // nx
+        X = np.broadcast_to(X1[:, None, jx], (Np, Np, len(e)))      # X[i, j, e] = X1[i, jx]
+        Y = np.broadcast_to(Y1[None, :, jy], (Np, Np, len(e)))      # Y[i, j, e] = Y1[j, jy]
+        return np.asfortranarray(X), np.asfortranarray(Y)
+
+
+def is_periodic(bc: AbstractBC) -> bool:
+    return isinstance(bc, PeriodicBC)
+
+
+@dataclass
+class MeshMetrics2D:
+    """Metric arrays at solution points and faces (MeshMetrics2D.jl:15-39), affine branch."""
+
+    x: np.ndarray
+    y: np.ndarray
+    x_xi: np.ndarray
+    x_eta: np.ndarray
+    y_xi: np.ndarray
+    y_eta: np.ndarray
+    J: np.ndarray
+    h_char: np.ndarray
+
+
+def build_mesh_metrics(mesh: Mesh2D, ops: FROperators) -> MeshMetrics2D:
+    """Volume metrics by applying D to the SP coordinates, in the reference's summation
+    order (MeshMetrics2D.jl:82-122), and affine h_char = min(dx, dy) (:206-209).
+
+    Face normals are not stored on the host: the device library rebuilds them from the
+    vertices and operators it is given.
+    """
+    Np = ops.n_points
+    X, Y = mesh.sp_coordinates(ops)
+    D = ops.D
+    shape = X.shape
+    xxi = np.zeros(shape, order="F")
+    yxi = np.zeros(shape, order="F")
+    xeta = np.zeros(shape, order="F")
+    yeta = np.zeros(shape, order="F")
+    for i in range(Np):
+        for k in range(Np):
+            xxi[i] = xxi[i] + D[i, k] * X[k]
+            yxi[i] = yxi[i] + D[i, k] * Y[k]
+    for j in range(Np):
+        for k in range(Np):
+            xeta[:, j] = xeta[:, j] + D[j, k] * X[:, k]
+            yeta[:, j] = yeta[:, j] + D[j, k] * Y[:, k]
+    J = xxi * yeta - xeta * yxi
+    jx = np.arange(mesh.n_elements) % mesh.nx
+    jy = np.arange(mesh.n_elements) // mesh.nx
+    h_char = np.minimum(mesh.dx[jx], mesh.dy[jy])
+    return MeshMetrics2D(X, Y, xxi, xeta, yxi, yeta, J, h_char)

@@ -1,0 +1,1007 @@
+/*
+ * frf_oracle.c -- CPU restatement (plain C, -ffp-contract=off) of FRForge's 2D FR residual,
+ * Persson sensor, BR0 / local-DD artificial viscosity and SSP-RK3 stepping.
+ *
+ * TEST INFRASTRUCTURE: the checker for the HIP product path.  Nothing in the product links it.
+ *
+ * Each function follows the reference Julia source line by line (same loop order, same
+ * operation order, no fused multiply-add).  Citations are to RcktMan77/FRForge.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   operators           Points.jl/Correction.jl/Operators.jl invariants (test_operators.jl:5-60)
+ *                       and numpy's Gauss-Legendre rule; node bits differ from the reference's
+ *                       LAPACK Golub-Welsch at the last ulp (reference's LAPACK: parity
+ *                       unpinned at ulp level, effect on du << 1e-12).
+ *   metrics             closed form of the affine map (x_xi = Jx etc. to roundoff).
+ *   residual + RK3      isentropic-vortex known answers, test_2d.jl:316-321 (rtol 1e-12),
+ *                       free-stream < 1e-12 (test_2d.jl:304-315), periodic mass telescoping.
+ *   Rusanov / HLLC      consistency F(u,u)=F(u).n, HLLC stationary contact (test_schemes.jl:88-103),
+ *                       an independent numpy HLLC on random states.
+ *   Persson sensor      sigma range / trigger on a jump (test_2d.jl:44-66), closed forms for
+ *                       pure-low-mode and pure-top-mode densities, numpy modal transform.
+ *   BR0                 viscous mass on constants < 1e-10 (test_2d.jl:47), periodic telescoping
+ *                       with variable eps, exactness on linear fields; an independent numpy
+ *                       BR0 on tiny meshes.
+ *   local DD            closed form on quadratic fields.
+ */
+#include "frf_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NEQ 4
+#define EPS DBL_EPSILON
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+
+/* Julia's max/min on Float64 propagate NaN. */
+static inline double jmax(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    return a > b ? a : (b > a ? b : (signbit(a) ? b : a));
+}
+static inline double jmin(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    return a < b ? a : (b < a ? b : (signbit(a) ? a : b));
+}
+
+/* ---------------------------------------------------------------- operators */
+
+/* legendre_P (Correction.jl:8-24) */
+static double legendre_P(int k, double x) {
+    if (k == 0) return 1.0;
+    if (k == 1) return x;
+    double pm2 = 1.0, pm1 = x, pk = pm1;
+    for (int m = 2; m <= k; ++m) {
+        pk = (((double)(2 * m - 1) * x * pm1) - ((double)(m - 1) * pm2)) / (double)m;
+        pm2 = pm1;
+        pm1 = pk;
+    }
+    return pk;
+}
+
+/* legendre_P_prime (Correction.jl:32-45) */
+static double legendre_P_prime(int k, double x) {
+    if (k == 0) return 0.0;
+    if (x >= 1.0 - 1000 * EPS) return (double)(k * (k + 1)) / 2.0;
+    if (x <= -1.0 + 1000 * EPS) return ((k % 2) ? 1.0 : -1.0) * (double)(k * (k + 1)) / 2.0;
+    double pk = legendre_P(k, x), pkm1 = legendre_P(k - 1, x);
+    return ((double)k * pkm1 - (double)k * x * pk) / (1.0 - x * x);
+}
+
+/* Gauss-Legendre nodes/weights.  The reference uses Golub-Welsch (Points.jl:10-33); this
+ * restatement uses Newton on P_n from Chebyshev guesses with weights 2/((1-x^2)P_n'^2),
+ * then the reference's normalisation w .*= 2/sum(w) (Points.jl:32). */
+static void gl_nodes(int n, double* x, double* w) {
+    if (n == 1) { x[0] = 0.0; w[0] = 2.0; return; }
+    for (int i = 0; i < n; ++i) {
+        double z = -cos(M_PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int m = 2; m <= n; ++m) {
+                double p2 = ((2.0 * m - 1.0) * z * p1 - (m - 1.0) * p0) / m;
+                p0 = p1; p1 = p2;
+            }
+            double dp = n * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-17) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int m = 2; m <= n; ++m) {
+            double p2 = ((2.0 * m - 1.0) * z * p1 - (m - 1.0) * p0) / m;
+            p0 = p1; p1 = p2;
+        }
+        double dp = n * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    /* symmetrise and sort ascending */
+    for (int i = 0; i < n / 2; ++i) {
+        double a = 0.5 * (fabs(x[i]) + fabs(x[n - 1 - i]));
+        x[i] = -a; x[n - 1 - i] = a;
+        double b = 0.5 * (w[i] + w[n - 1 - i]);
+        w[i] = b; w[n - 1 - i] = b;
+    }
+    if (n % 2) x[n / 2] = 0.0;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += w[i];
+    for (int i = 0; i < n; ++i) w[i] *= 2.0 / s;
+}
+
+int orc_build_operators(int p, double* xi, double* w, double* D, double* lL, double* lR,
+                        double* gLxi, double* gRxi, double* gL, double* gR, double* V) {
+    if (p < 0) return -1;
+    int n = p + 1;
+    gl_nodes(n, xi, w);
+    /* differentiation_matrix (Points.jl:83-104); D[i,j] at D[i + n*j] */
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            if (i == j) { D[i + n * j] = 0.0; continue; }
+            double num = 1.0, den = xi[j] - xi[i];
+            for (int k = 0; k < n; ++k)
+                if (k != i && k != j) num *= (xi[i] - xi[k]) / (xi[j] - xi[k]);
+            D[i + n * j] = num / den;
+        }
+        double s = 0.0;
+        int first = 1;
+        for (int k = 0; k < n; ++k) {
+            if (k == i) continue;
+            if (first) { s = D[i + n * k]; first = 0; } else s += D[i + n * k];
+        }
+        D[i + n * i] = -s;
+    }
+    /* lagrange_at (Points.jl:65-76) */
+    for (int j = 0; j < n; ++j) {
+        double a = 1.0, b = 1.0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            a *= (-1.0 - xi[k]) / (xi[j] - xi[k]);
+            b *= (1.0 - xi[k]) / (xi[j] - xi[k]);
+        }
+        lL[j] = a; lR[j] = b;
+    }
+    /* g_DG_values_and_derivs (Correction.jl:57-86) */
+    double rRm1 = legendre_P(p + 1, -1.0) - legendre_P(p, -1.0);
+    double rLp1 = legendre_P(p + 1, 1.0) + legendre_P(p, 1.0);
+    for (int i = 0; i < n; ++i) {
+        double x = xi[i];
+        double Pp = legendre_P(p, x), Pp1 = legendre_P(p + 1, x);
+        gL[i] = (Pp1 - Pp) / rRm1;
+        gR[i] = (Pp1 + Pp) / rLp1;
+        double dPp = legendre_P_prime(p, x), dPp1 = legendre_P_prime(p + 1, x);
+        gLxi[i] = (dPp1 - dPp) / rRm1;
+        gRxi[i] = (dPp1 + dPp) / rLp1;
+    }
+    /* legendre_vandermonde (Correction.jl:93-100): V[j,k] = P_{k-1}(xi_j) */
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) V[j + n * k] = legendre_P(k, xi[j]);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- context */
+
+struct orc_ctx {
+    orc_problem pb;
+    int Np, Nel;
+    double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
+    double *LU; int* piv;             /* LU of V with partial pivoting (sensor solves) */
+    double *xv, *yv, *dx, *dy, *Jx, *Jy;
+    double* met[20];
+    /* workspace */
+    double *trW, *trE, *trS, *trN, *fW, *fE, *fS, *fN, *sigma, *eps;
+    double *gx, *gy, *uW, *uE, *uS, *uN, *gxW, *gxE, *gyS, *gyN, *ghW, *ghE, *ghS, *ghN;
+    double *du, *u0, *u1, *u2;
+};
+
+#define IDX4(i, j, e, c) ((i) + Np * ((j) + Np * ((e) + Nel * (c))))
+#define IDX3(i, j, e) ((i) + Np * ((j) + Np * (e)))
+#define FIDX(q, c, e) ((q) + Np * ((c) + NEQ * (e)))   /* (Np,Neq,Nel) */
+#define BIDX(q, e, c) ((q) + Np * ((e) + Nel * (c)))   /* (Np,Nel,Neq) */
+#define MIDX(q, e) ((q) + Np * (e))
+
+static double* dup_arr(const double* s, size_t n) {
+    double* d = (double*)malloc(n * sizeof(double));
+    memcpy(d, s, n * sizeof(double));
+    return d;
+}
+
+/* physical_xy for the affine map (MeshMetrics2D.jl:49-58) */
+static void physical_xy(const orc_ctx* h, int e, double xi, double eta, double* X, double* Y) {
+    int nx = h->pb.nx;
+    int jx = e % nx, jy = e / nx;
+    double xL = h->xv[jx], xR = h->xv[jx + 1], yB = h->yv[jy], yT = h->yv[jy + 1];
+    *X = (xL + xR) / 2.0 + h->Jx[jx] * xi;
+    *Y = (yB + yT) / 2.0 + h->Jy[jy] * eta;
+}
+
+/* build_mesh_metrics, affine branch (MeshMetrics2D.jl:82-218) */
+static void build_metrics(orc_ctx* h) {
+    int Np = h->Np, Nel = h->Nel;
+    size_t nv = (size_t)Np * Np * Nel, nf = (size_t)Np * Nel;
+    for (int k = 0; k < 7; ++k) h->met[k] = (double*)calloc(nv, sizeof(double));
+    for (int k = 7; k < 19; ++k) h->met[k] = (double*)calloc(nf, sizeof(double));
+    h->met[19] = (double*)calloc(Nel, sizeof(double));
+    double *x = h->met[0], *y = h->met[1], *xxi = h->met[2], *xeta = h->met[3];
+    double *yxi = h->met[4], *yeta = h->met[5], *J = h->met[6];
+    const double* D = h->D;
+    for (int e = 0; e < Nel; ++e) {
+        for (int j = 0; j < Np; ++j)
+            for (int i = 0; i < Np; ++i)
+                physical_xy(h, e, h->xi[i], h->xi[j], &x[IDX3(i, j, e)], &y[IDX3(i, j, e)]);
+        for (int j = 0; j < Np; ++j)
+            for (int i = 0; i < Np; ++i) {
+                double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+                for (int k = 0; k < Np; ++k) {
+                    a += D[i + Np * k] * x[IDX3(k, j, e)];
+                    b += D[i + Np * k] * y[IDX3(k, j, e)];
+                    c += D[j + Np * k] * x[IDX3(i, k, e)];
+                    d += D[j + Np * k] * y[IDX3(i, k, e)];
+                }
+                xxi[IDX3(i, j, e)] = a;
+                yxi[IDX3(i, j, e)] = b;
+                xeta[IDX3(i, j, e)] = c;
+                yeta[IDX3(i, j, e)] = d;
+                J[IDX3(i, j, e)] = a * d - c * b;
+            }
+    }
+    double *nxW = h->met[7], *nyW = h->met[8], *sJW = h->met[9];
+    double *nxE = h->met[10], *nyE = h->met[11], *sJE = h->met[12];
+    double *nxS = h->met[13], *nyS = h->met[14], *sJS = h->met[15];
+    double *nxN = h->met[16], *nyN = h->met[17], *sJN = h->met[18];
+    const double *lL = h->lL, *lR = h->lR;
+    for (int e = 0; e < Nel; ++e) {
+        for (int j = 0; j < Np; ++j) {
+            double xW = 0, yW = 0, xE = 0, yE = 0;
+            for (int i = 0; i < Np; ++i) {
+                xW += lL[i] * xeta[IDX3(i, j, e)];
+                yW += lL[i] * yeta[IDX3(i, j, e)];
+                xE += lR[i] * xeta[IDX3(i, j, e)];
+                yE += lR[i] * yeta[IDX3(i, j, e)];
+            }
+            double sE = sqrt(yE * yE + xE * xE);
+            sE = jmax(sE, EPS);
+            sJE[MIDX(j, e)] = sE; nxE[MIDX(j, e)] = yE / sE; nyE[MIDX(j, e)] = -xE / sE;
+            double sW = sqrt(yW * yW + xW * xW);
+            sW = jmax(sW, EPS);
+            sJW[MIDX(j, e)] = sW; nxW[MIDX(j, e)] = -yW / sW; nyW[MIDX(j, e)] = xW / sW;
+        }
+        for (int i = 0; i < Np; ++i) {
+            double xS = 0, yS = 0, xN = 0, yN = 0;
+            for (int j = 0; j < Np; ++j) {
+                xS += lL[j] * xxi[IDX3(i, j, e)];
+                yS += lL[j] * yxi[IDX3(i, j, e)];
+                xN += lR[j] * xxi[IDX3(i, j, e)];
+                yN += lR[j] * yxi[IDX3(i, j, e)];
+            }
+            double sN = sqrt(yN * yN + xN * xN);
+            sN = jmax(sN, EPS);
+            sJN[MIDX(i, e)] = sN; nxN[MIDX(i, e)] = -yN / sN; nyN[MIDX(i, e)] = xN / sN;
+            double sS = sqrt(yS * yS + xS * xS);
+            sS = jmax(sS, EPS);
+            sJS[MIDX(i, e)] = sS; nxS[MIDX(i, e)] = yS / sS; nyS[MIDX(i, e)] = -xS / sS;
+        }
+    }
+    int nx = h->pb.nx;
+    for (int e = 0; e < Nel; ++e) {
+        int jx = e % nx, jy = e / nx;
+        h->met[19][e] = jmin(h->dx[jx], h->dy[jy]);   /* affine h_char (MeshMetrics2D.jl:206-209) */
+    }
+}
+
+/* LU with partial pivoting of V (column-major), used for the sensor's V \ U solves. */
+static void lu_factor(const double* A, int n, double* LU, int* piv) {
+    memcpy(LU, A, sizeof(double) * n * n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double m = fabs(LU[k + n * k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(LU[i + n * k]) > m) { m = fabs(LU[i + n * k]); p = i; }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; ++j) {
+                double t = LU[k + n * j]; LU[k + n * j] = LU[p + n * j]; LU[p + n * j] = t;
+            }
+        for (int i = k + 1; i < n; ++i) {
+            LU[i + n * k] /= LU[k + n * k];
+            for (int j = k + 1; j < n; ++j) LU[i + n * j] -= LU[i + n * k] * LU[k + n * j];
+        }
+    }
+}
+/* Solve V x = b in place. */
+static void lu_solve(const double* LU, const int* piv, int n, double* b) {
+    for (int k = 0; k < n; ++k) {
+        int p = piv[k];
+        if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < i; ++k) b[i] -= LU[i + n * k] * b[k];
+    for (int i = n - 1; i >= 0; --i) {
+        for (int k = i + 1; k < n; ++k) b[i] -= LU[i + n * k] * b[k];
+        b[i] /= LU[i + n * i];
+    }
+}
+
+orc_ctx* orc_create(const orc_problem* pb, const double* xi, const double* w, const double* D,
+                    const double* lL, const double* lR, const double* gLxi, const double* gRxi,
+                    const double* V, const double* xv, const double* yv) {
+    if (pb->p < 0 || pb->nx < 1 || pb->ny < 1) return NULL;
+    orc_ctx* h = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+    h->pb = *pb;
+    int Np = pb->p + 1, Nel = pb->nx * pb->ny;
+    h->Np = Np; h->Nel = Nel;
+    h->xi = dup_arr(xi, Np); h->w = dup_arr(w, Np); h->D = dup_arr(D, Np * Np);
+    h->lL = dup_arr(lL, Np); h->lR = dup_arr(lR, Np);
+    h->gLxi = dup_arr(gLxi, Np); h->gRxi = dup_arr(gRxi, Np); h->V = dup_arr(V, Np * Np);
+    h->LU = (double*)malloc(sizeof(double) * Np * Np);
+    h->piv = (int*)malloc(sizeof(int) * Np);
+    lu_factor(h->V, Np, h->LU, h->piv);
+    h->xv = dup_arr(xv, pb->nx + 1); h->yv = dup_arr(yv, pb->ny + 1);
+    h->dx = (double*)malloc(sizeof(double) * pb->nx);
+    h->dy = (double*)malloc(sizeof(double) * pb->ny);
+    h->Jx = (double*)malloc(sizeof(double) * pb->nx);
+    h->Jy = (double*)malloc(sizeof(double) * pb->ny);
+    for (int k = 0; k < pb->nx; ++k) { h->dx[k] = xv[k + 1] - xv[k]; h->Jx[k] = h->dx[k] / 2.0; }
+    for (int k = 0; k < pb->ny; ++k) { h->dy[k] = yv[k + 1] - yv[k]; h->Jy[k] = h->dy[k] / 2.0; }
+    build_metrics(h);
+    size_t nf = (size_t)Np * NEQ * Nel, ns = (size_t)Np * Np * Nel * NEQ;
+    double** fa[] = {&h->trW, &h->trE, &h->trS, &h->trN, &h->fW, &h->fE, &h->fS, &h->fN,
+                     &h->uW, &h->uE, &h->uS, &h->uN, &h->gxW, &h->gxE, &h->gyS, &h->gyN,
+                     &h->ghW, &h->ghE, &h->ghS, &h->ghN};
+    for (size_t k = 0; k < sizeof(fa) / sizeof(fa[0]); ++k) *fa[k] = (double*)calloc(nf, sizeof(double));
+    double** sa[] = {&h->gx, &h->gy, &h->du, &h->u0, &h->u1, &h->u2};
+    for (size_t k = 0; k < sizeof(sa) / sizeof(sa[0]); ++k) *sa[k] = (double*)calloc(ns, sizeof(double));
+    h->sigma = (double*)calloc(Nel, sizeof(double));
+    h->eps = (double*)calloc(Nel, sizeof(double));
+    return h;
+}
+
+void orc_destroy(orc_ctx* h) {
+    if (!h) return;
+    double* all[] = {h->xi, h->w, h->D, h->lL, h->lR, h->gLxi, h->gRxi, h->V, h->LU, h->xv, h->yv,
+                     h->dx, h->dy, h->Jx, h->Jy, h->trW, h->trE, h->trS, h->trN, h->fW, h->fE,
+                     h->fS, h->fN, h->sigma, h->eps, h->gx, h->gy, h->uW, h->uE, h->uS, h->uN,
+                     h->gxW, h->gxE, h->gyS, h->gyN, h->ghW, h->ghE, h->ghS, h->ghN, h->du,
+                     h->u0, h->u1, h->u2};
+    for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); ++k) free(all[k]);
+    for (int k = 0; k < 20; ++k) free(h->met[k]);
+    free(h->piv);
+    free(h);
+}
+
+const double* orc_metric(orc_ctx* h, int which) {
+    return (which >= 0 && which < 20) ? h->met[which] : NULL;
+}
+
+/* ---------------------------------------------------------------- physics */
+
+/* pressure (Euler2D.jl:14-19) */
+static inline double pressure(double g, const double* U) {
+    double rho = jmax(U[0], EPS);
+    double u = U[1] / rho, v = U[2] / rho;
+    return (g - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v));
+}
+/* sound_speed (Euler2D.jl:35-39) */
+static inline double sound_speed(double g, const double* U) {
+    double rho = jmax(U[0], EPS);
+    double p = jmax(pressure(g, U), EPS);
+    return sqrt(g * p / rho);
+}
+/* physical_flux_x! / physical_flux_y! (Euler2D.jl:54-79) */
+static inline void flux_x(double g, const double* U, double* out) {
+    double u = U[1] / jmax(U[0], EPS);
+    double p = pressure(g, U);
+    out[0] = U[1]; out[1] = U[1] * u + p; out[2] = U[2] * u; out[3] = (U[3] + p) * u;
+}
+static inline void flux_y(double g, const double* U, double* out) {
+    double v = U[2] / jmax(U[0], EPS);
+    double p = pressure(g, U);
+    out[0] = U[2]; out[1] = U[1] * v; out[2] = U[2] * v + p; out[3] = (U[3] + p) * v;
+}
+/* max_wave_speed_n (Euler2D.jl:81-90) */
+static inline double spd_n(double g, const double* U, double nx, double ny) {
+    double rho = jmax(U[0], EPS);
+    double u = U[1] / rho, v = U[2] / rho;
+    double un = u * nx + v * ny;
+    return fabs(un) + sound_speed(g, U);
+}
+/* numerical_flux_n! Rusanov (Euler2D.jl:114-151) */
+static void rusanov_n(double g, const double* uL, const double* uR, double nx, double ny, double* out) {
+    double rL = jmax(uL[0], EPS), uL_ = uL[1] / rL, vL_ = uL[2] / rL, pL = pressure(g, uL);
+    double rR = jmax(uR[0], EPS), uR_ = uR[1] / rR, vR_ = uR[2] / rR, pR = pressure(g, uR);
+    double fL1 = uL[1] * nx + uL[2] * ny;
+    double fL2 = (uL[1] * uL_ + pL) * nx + (uL[1] * vL_) * ny;
+    double fL3 = (uL[2] * uL_) * nx + (uL[2] * vL_ + pL) * ny;
+    double fL4 = (uL[3] + pL) * (uL_ * nx + vL_ * ny);
+    double fR1 = uR[1] * nx + uR[2] * ny;
+    double fR2 = (uR[1] * uR_ + pR) * nx + (uR[1] * vR_) * ny;
+    double fR3 = (uR[2] * uR_) * nx + (uR[2] * vR_ + pR) * ny;
+    double fR4 = (uR[3] + pR) * (uR_ * nx + vR_ * ny);
+    double lam = jmax(spd_n(g, uL, nx, ny), spd_n(g, uR, nx, ny));
+    out[0] = 0.5 * (fL1 + fR1) - 0.5 * lam * (uR[0] - uL[0]);
+    out[1] = 0.5 * (fL2 + fR2) - 0.5 * lam * (uR[1] - uL[1]);
+    out[2] = 0.5 * (fL3 + fR3) - 0.5 * lam * (uR[2] - uL[2]);
+    out[3] = 0.5 * (fL4 + fR4) - 0.5 * lam * (uR[3] - uL[3]);
+}
+/* hllc_flux_n (HLLC.jl:235-305) */
+static void hllc_n(double g, const double* uL, const double* uR, double nx0, double ny0, double* out) {
+    double nrm = hypot(nx0, ny0);
+    if (nrm < EPS) { rusanov_n(g, uL, uR, nx0, ny0, out); return; }
+    double nx = nx0 / nrm, ny = ny0 / nrm;
+    double rL = jmax(uL[0], EPS), ulv = uL[1] / rL, vlv = uL[2] / rL;
+    double pL = jmax(pressure(g, uL), EPS), unL = ulv * nx + vlv * ny;
+    double rR = jmax(uR[0], EPS), urv = uR[1] / rR, vrv = uR[2] / rR;
+    double pR = jmax(pressure(g, uR), EPS), unR = urv * nx + vrv * ny;
+    double cL = sqrt(g * pL / rL), cR = sqrt(g * pR / rR);
+    double SL = jmin(unL - cL, unR - cR);
+    double SR = jmax(unL + cL, unR + cR);
+    double num = pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR);
+    double den = rL * (SL - unL) - rR * (SR - unR);
+    double SM = fabs(den) < EPS ? 0.5 * (unL + unR) : num / den;
+    double fx[4], fy[4], fL[4], fR[4];
+    flux_x(g, uL, fx); flux_y(g, uL, fy);
+    for (int c = 0; c < 4; ++c) fL[c] = fx[c] * nx + fy[c] * ny;
+    flux_x(g, uR, fx); flux_y(g, uR, fy);
+    for (int c = 0; c < 4; ++c) fR[c] = fx[c] * nx + fy[c] * ny;
+    if (0.0 <= SL) { memcpy(out, fL, sizeof fL); return; }
+    if (0.0 >= SR) { memcpy(out, fR, sizeof fR); return; }
+    const double *U, *F;
+    double rho, uu, vv, p, un, S;
+    if (SL <= 0.0 && 0.0 <= SM) { U = uL; F = fL; rho = rL; uu = ulv; vv = vlv; p = pL; un = unL; S = SL; }
+    else { U = uR; F = fR; rho = rR; uu = urv; vv = vrv; p = pR; un = unR; S = SR; }
+    double utx = uu - un * nx, uty = vv - un * ny;
+    double us = utx + SM * nx, vs = uty + SM * ny;
+    double factor = rho * (S - un) / (S - SM);
+    double es = U[3] / rho + (SM - un) * (SM + p / (rho * (S - un)));
+    double Us[4] = {factor, factor * us, factor * vs, factor * es};
+    for (int c = 0; c < 4; ++c) out[c] = F[c] + S * (Us[c] - U[c]);
+}
+static inline void iflux(const orc_ctx* h, const double* uL, const double* uR, double nx, double ny, double* out) {
+    if (h->pb.flux == ORC_FLUX_HLLC) hllc_n(h->pb.gamma, uL, uR, nx, ny, out);
+    else rusanov_n(h->pb.gamma, uL, uR, nx, ny, out);
+}
+void orc_flux_n(int kind, double gamma, const double* uL, const double* uR, double nx, double ny,
+                double* out) {
+    if (kind == ORC_FLUX_HLLC) hllc_n(gamma, uL, uR, nx, ny, out);
+    else rusanov_n(gamma, uL, uR, nx, ny, out);
+}
+
+/* ---------------------------------------------------------------- residual */
+
+/* boundary_fhat! for TransmissiveBC (Residual2D.jl:63-94; ghost = interior, BoundaryConditions.jl:67-77) */
+static void boundary_fhat(const orc_ctx* h, double* fhat, int q, int e, const double* uint_,
+                          double nxo, double nyo, double sJ, int into_domain) {
+    int Np = h->Np;
+    double ug[4], fh[4];
+    memcpy(ug, uint_, sizeof ug);
+    if (into_domain) iflux(h, ug, uint_, -nxo, -nyo, fh);
+    else iflux(h, uint_, ug, nxo, nyo, fh);
+    for (int c = 0; c < 4; ++c) fhat[FIDX(q, c, e)] = fh[c] * sJ;
+}
+
+static void get_face(const orc_ctx* h, const double* tr, int e, int q, double* out) {
+    int Np = h->Np;
+    for (int c = 0; c < 4; ++c) out[c] = tr[FIDX(q, c, e)];
+}
+
+static void apply_br0(orc_ctx* h, const double* u, const double* eps, double* du);
+static void apply_dd(orc_ctx* h, const double* u, const double* eps, double* du);
+
+void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
+    (void)t;
+    const int Np = h->Np, Nel = h->Nel, nx = h->pb.nx, ny = h->pb.ny;
+    const double *D = h->D, *gLv = h->gLxi, *gRv = h->gRxi, *lL = h->lL, *lR = h->lR;
+    const double g = h->pb.gamma;
+    double **met = h->met;
+    size_t ns = (size_t)Np * Np * Nel * NEQ;
+    memset(du, 0, ns * sizeof(double));   /* fill!(du, 0) (:150) */
+    double *trW = h->trW, *trE = h->trE, *trS = h->trS, *trN = h->trN;
+    double *fW = h->fW, *fE = h->fE, *fS = h->fS, *fN = h->fN;
+
+    /* face traces (:157-164, face_trace! :9-48) */
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int e = 0; e < Nel; ++e) {
+        for (int j = 0; j < Np; ++j)
+            for (int c = 0; c < NEQ; ++c) {
+                double a = 0.0, b = 0.0;
+                for (int i = 0; i < Np; ++i) a += lL[i] * u[IDX4(i, j, e, c)];
+                for (int i = 0; i < Np; ++i) b += lR[i] * u[IDX4(i, j, e, c)];
+                trW[FIDX(j, c, e)] = a;
+                trE[FIDX(j, c, e)] = b;
+            }
+        for (int i = 0; i < Np; ++i)
+            for (int c = 0; c < NEQ; ++c) {
+                double a = 0.0, b = 0.0;
+                for (int j = 0; j < Np; ++j) a += lL[j] * u[IDX4(i, j, e, c)];
+                for (int j = 0; j < Np; ++j) b += lR[j] * u[IDX4(i, j, e, c)];
+                trS[FIDX(i, c, e)] = a;
+                trN[FIDX(i, c, e)] = b;
+            }
+    }
+
+    /* interior vertical faces (:173-212) */
+    int nvf = ny * (nx - 1);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int f = 0; f < nvf; ++f) {
+        int jy = f / (nx - 1), jx = f % (nx - 1);
+        int eL = jy * nx + jx, eR = eL + 1;
+        double um[4], up[4], fh[4];
+        for (int q = 0; q < Np; ++q) {
+            double nnx = met[10][MIDX(q, eL)], nny = met[11][MIDX(q, eL)], sJ = met[12][MIDX(q, eL)];
+            get_face(h, trE, eL, q, um);
+            get_face(h, trW, eR, q, up);
+            iflux(h, um, up, nnx, nny, fh);
+            for (int c = 0; c < 4; ++c) {
+                double val = fh[c] * sJ;
+                fE[FIDX(q, c, eL)] = val;
+                fW[FIDX(q, c, eR)] = val;
+            }
+        }
+    }
+    /* domain left/right (:215-252) */
+    for (int jy = 0; jy < ny; ++jy) {
+        int eL = jy * nx, eR = jy * nx + nx - 1;
+        double um[4], up[4], fh[4];
+        for (int q = 0; q < Np; ++q) {
+            if (h->pb.bc_x == ORC_BC_PERIODIC) {
+                get_face(h, trE, eR, q, um);
+                get_face(h, trW, eL, q, up);
+                double nnx = met[10][MIDX(q, eR)], nny = met[11][MIDX(q, eR)], sJ = met[12][MIDX(q, eR)];
+                iflux(h, um, up, nnx, nny, fh);
+                for (int c = 0; c < 4; ++c) {
+                    double val = fh[c] * sJ;
+                    fE[FIDX(q, c, eR)] = val;
+                    fW[FIDX(q, c, eL)] = val;
+                }
+            } else {
+                get_face(h, trW, eL, q, um);
+                boundary_fhat(h, fW, q, eL, um, met[7][MIDX(q, eL)], met[8][MIDX(q, eL)],
+                              met[9][MIDX(q, eL)], 1);
+                get_face(h, trE, eR, q, up);
+                boundary_fhat(h, fE, q, eR, up, met[10][MIDX(q, eR)], met[11][MIDX(q, eR)],
+                              met[12][MIDX(q, eR)], 0);
+            }
+        }
+    }
+    /* interior horizontal faces (:254-294) */
+    int nhf = (ny - 1) * nx;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int f = 0; f < nhf; ++f) {
+        int jy = f / nx, jx = f % nx;
+        int eB = jy * nx + jx, eT = eB + nx;
+        double um[4], up[4], fh[4];
+        for (int q = 0; q < Np; ++q) {
+            double nnx = met[16][MIDX(q, eB)], nny = met[17][MIDX(q, eB)], sJ = met[18][MIDX(q, eB)];
+            get_face(h, trN, eB, q, um);
+            get_face(h, trS, eT, q, up);
+            iflux(h, um, up, nnx, nny, fh);
+            for (int c = 0; c < 4; ++c) {
+                double val = fh[c] * sJ;
+                fN[FIDX(q, c, eB)] = val;
+                fS[FIDX(q, c, eT)] = val;
+            }
+        }
+    }
+    /* domain bottom/top (:297-334) */
+    for (int jx = 0; jx < nx; ++jx) {
+        int eB = jx, eT = (ny - 1) * nx + jx;
+        double um[4], up[4], fh[4];
+        for (int q = 0; q < Np; ++q) {
+            if (h->pb.bc_y == ORC_BC_PERIODIC) {
+                get_face(h, trN, eT, q, um);
+                get_face(h, trS, eB, q, up);
+                double nnx = met[16][MIDX(q, eT)], nny = met[17][MIDX(q, eT)], sJ = met[18][MIDX(q, eT)];
+                iflux(h, um, up, nnx, nny, fh);
+                for (int c = 0; c < 4; ++c) {
+                    double val = fh[c] * sJ;
+                    fN[FIDX(q, c, eT)] = val;
+                    fS[FIDX(q, c, eB)] = val;
+                }
+            } else {
+                get_face(h, trS, eB, q, um);
+                boundary_fhat(h, fS, q, eB, um, met[13][MIDX(q, eB)], met[14][MIDX(q, eB)],
+                              met[15][MIDX(q, eB)], 1);
+                get_face(h, trN, eT, q, up);
+                boundary_fhat(h, fN, q, eT, up, met[16][MIDX(q, eT)], met[17][MIDX(q, eT)],
+                              met[18][MIDX(q, eT)], 0);
+            }
+        }
+    }
+
+    /* volume (:337-398) */
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* Ft = (double*)malloc(sizeof(double) * Np * Np * NEQ);
+        double* Gt = (double*)malloc(sizeof(double) * Np * Np * NEQ);
+#pragma omp for schedule(static)
+        for (int e = 0; e < Nel; ++e) {
+            double U[4], Fx[4], Gy[4];
+            for (int j = 0; j < Np; ++j)
+                for (int i = 0; i < Np; ++i) {
+                    for (int c = 0; c < 4; ++c) U[c] = u[IDX4(i, j, e, c)];
+                    flux_x(g, U, Fx);
+                    flux_y(g, U, Gy);
+                    double xxi = met[2][IDX3(i, j, e)], xeta = met[3][IDX3(i, j, e)];
+                    double yxi = met[4][IDX3(i, j, e)], yeta = met[5][IDX3(i, j, e)];
+                    for (int c = 0; c < 4; ++c) {
+                        Ft[i + Np * (j + Np * c)] = yeta * Fx[c] - xeta * Gy[c];
+                        Gt[i + Np * (j + Np * c)] = -yxi * Fx[c] + xxi * Gy[c];
+                    }
+                }
+            for (int j = 0; j < Np; ++j)
+                for (int c = 0; c < NEQ; ++c) {
+                    double fL = 0.0, fR = 0.0;
+                    for (int i = 0; i < Np; ++i) {
+                        fL += lL[i] * Ft[i + Np * (j + Np * c)];
+                        fR += lR[i] * Ft[i + Np * (j + Np * c)];
+                    }
+                    for (int i = 0; i < Np; ++i) {
+                        double vol = 0.0;
+                        for (int k = 0; k < Np; ++k) vol += D[i + Np * k] * Ft[k + Np * (j + Np * c)];
+                        double corr = (fW[FIDX(j, c, e)] - fL) * gLv[i] + (fE[FIDX(j, c, e)] - fR) * gRv[i];
+                        double Je = met[6][IDX3(i, j, e)];
+                        du[IDX4(i, j, e, c)] -= (vol + corr) / Je;
+                    }
+                }
+            for (int i = 0; i < Np; ++i)
+                for (int c = 0; c < NEQ; ++c) {
+                    double gS = 0.0, gN = 0.0;
+                    for (int j = 0; j < Np; ++j) {
+                        gS += lL[j] * Gt[i + Np * (j + Np * c)];
+                        gN += lR[j] * Gt[i + Np * (j + Np * c)];
+                    }
+                    for (int j = 0; j < Np; ++j) {
+                        double vol = 0.0;
+                        for (int k = 0; k < Np; ++k) vol += D[j + Np * k] * Gt[i + Np * (k + Np * c)];
+                        double corr = (fS[FIDX(i, c, e)] - gS) * gLv[j] + (fN[FIDX(i, c, e)] - gN) * gRv[j];
+                        double Je = met[6][IDX3(i, j, e)];
+                        du[IDX4(i, j, e, c)] -= (vol + corr) / Je;
+                    }
+                }
+        }
+        free(Ft);
+        free(Gt);
+    }
+
+    /* sense! + apply_dissipation! (:400-407) */
+    if (h->pb.capturing == ORC_CAPT_NULL) return;
+    orc_sense(h, u, h->sigma);
+    orc_element_viscosities(h, u, h->sigma, h->eps);
+    if (h->pb.capturing == ORC_CAPT_PERSSON_BR0) apply_br0(h, u, h->eps, du);
+    else apply_dd(h, u, h->eps, du);
+}
+
+/* ---------------------------------------------------------------- capturing */
+
+/* persson_s0 / persson_sigma (SharedAV.jl:9-21) */
+static inline double persson_sigma(double e_high, double e_tot, double s0, double kappa, double floor_) {
+    double s_e = log10(e_high / e_tot + floor_);
+    return 1.0 / (1.0 + exp(-kappa * (s_e - s0)));
+}
+
+/* sense! 2D (PerssonAV2D.jl:20-59): U^ = V^{-1} U V^{-T} via two LU solves */
+void orc_sense(orc_ctx* h, const double* u, double* sigma) {
+    const int Np = h->Np, Nel = h->Nel;
+    const int p = h->pb.p;
+    double s0 = h->pb.s0_factor * log10((double)(p > 1 ? p : 1));
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* A = (double*)malloc(sizeof(double) * Np * Np);
+        double* col = (double*)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+        for (int e = 0; e < Nel; ++e) {
+            for (int j = 0; j < Np; ++j)
+                for (int i = 0; i < Np; ++i) A[i + Np * j] = u[IDX4(i, j, e, 0)];
+            /* A <- V \ A (column by column) */
+            for (int j = 0; j < Np; ++j) {
+                for (int i = 0; i < Np; ++i) col[i] = A[i + Np * j];
+                lu_solve(h->LU, h->piv, Np, col);
+                for (int i = 0; i < Np; ++i) A[i + Np * j] = col[i];
+            }
+            /* A <- A / V' = (V \ A')' (row by row) */
+            for (int i = 0; i < Np; ++i) {
+                for (int j = 0; j < Np; ++j) col[j] = A[i + Np * j];
+                lu_solve(h->LU, h->piv, Np, col);
+                for (int j = 0; j < Np; ++j) A[i + Np * j] = col[j];
+            }
+            double e_tot = 0.0;
+            for (int k = 0; k < Np * Np; ++k) e_tot += A[k] * A[k];
+            e_tot += h->pb.eps_floor;
+            double e_high = 0.0;
+            for (int j = 0; j < Np; ++j) e_high += A[(Np - 1) + Np * j] * A[(Np - 1) + Np * j];
+            for (int i = 0; i < Np - 1; ++i) e_high += A[i + Np * (Np - 1)] * A[i + Np * (Np - 1)];
+            sigma[e] = persson_sigma(e_high, e_tot, s0, h->pb.kappa, h->pb.eps_floor);
+        }
+        free(A);
+        free(col);
+    }
+}
+
+/* element_max_wavespeed_2d (PerssonAV2D.jl:62-82) */
+static double elem_max_wavespeed(const orc_ctx* h, const double* u, int e) {
+    const int Np = h->Np, Nel = h->Nel;
+    const double g = h->pb.gamma;
+    double m = 0.0;
+    for (int j = 0; j < Np; ++j)
+        for (int i = 0; i < Np; ++i) {
+            double U[4];
+            for (int c = 0; c < 4; ++c) U[c] = u[IDX4(i, j, e, c)];
+            double rho = jmax(U[0], EPS);
+            double uu = U[1] / rho, vv = U[2] / rho;
+            double c = sound_speed(g, U);
+            m = jmax(jmax(m, fabs(uu) + c), fabs(vv) + c);
+        }
+    return jmax(m, EPS);
+}
+
+void orc_element_viscosities(orc_ctx* h, const double* u, const double* sigma, double* eps) {
+    const int Nel = h->Nel;
+    int p = h->pb.p > 1 ? h->pb.p : 1;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int e = 0; e < Nel; ++e) {
+        eps[e] = 0.0;
+        double se = sigma[e];
+        if (se <= 0.0) continue;
+        double hc = h->met[19][e];
+        double lam = elem_max_wavespeed(h, u, e);
+        eps[e] = h->pb.c_av * se * (hc / (double)p) * lam;
+    }
+}
+
+/* br0_tau_1d / br0_avg_flux (SharedAV.jl:29-42) */
+static inline double br0_tau(double em, double ep, double hm, double hp, int p) {
+    double ebar = 0.5 * (em + ep);
+    double hbar = 0.5 * (hm + hp);
+    return 2.0 * ebar * (double)(p + 1) / jmax(hbar, EPS);
+}
+static inline double br0_avg(double gm, double gp, double um, double up, double tau) {
+    return 0.5 * (gm + gp) - tau * (up - um);
+}
+
+/* apply_dissipation_br0_2d! (PerssonAV2D.jl:203-398) with eps given */
+static void apply_br0(orc_ctx* h, const double* u, const double* eps, double* du) {
+    const int Np = h->Np, Nel = h->Nel, nx = h->pb.nx, ny = h->pb.ny, p = h->pb.p;
+    const double *D = h->D, *lL = h->lL, *lR = h->lR, *gLv = h->gLxi, *gRv = h->gRxi;
+    double *gx = h->gx, *gy = h->gy;
+    double *uW = h->uW, *uE = h->uE, *uS = h->uS, *uN = h->uN;
+    double *gxW = h->gxW, *gxE = h->gxE, *gyS = h->gyS, *gyN = h->gyN;
+    double *ghW = h->ghW, *ghE = h->ghE, *ghS = h->ghS, *ghN = h->ghN;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int e = 0; e < Nel; ++e) {
+        int jx = e % nx, jy = e / nx;
+        double Jx = h->Jx[jx], Jy = h->Jy[jy], ee = eps[e];
+        for (int c = 0; c < NEQ; ++c) {
+            for (int j = 0; j < Np; ++j)
+                for (int i = 0; i < Np; ++i) {
+                    double dxi = 0.0, deta = 0.0;
+                    for (int k = 0; k < Np; ++k) {
+                        dxi += D[i + Np * k] * u[IDX4(k, j, e, c)];
+                        deta += D[j + Np * k] * u[IDX4(i, k, e, c)];
+                    }
+                    gx[IDX4(i, j, e, c)] = ee * dxi / Jx;
+                    gy[IDX4(i, j, e, c)] = ee * deta / Jy;
+                }
+            for (int j = 0; j < Np; ++j) {
+                double suW = 0, suE = 0, sgW = 0, sgE = 0;
+                for (int i = 0; i < Np; ++i) {
+                    suW += lL[i] * u[IDX4(i, j, e, c)];
+                    suE += lR[i] * u[IDX4(i, j, e, c)];
+                    sgW += lL[i] * gx[IDX4(i, j, e, c)];
+                    sgE += lR[i] * gx[IDX4(i, j, e, c)];
+                }
+                uW[BIDX(j, e, c)] = suW; uE[BIDX(j, e, c)] = suE;
+                gxW[BIDX(j, e, c)] = sgW; gxE[BIDX(j, e, c)] = sgE;
+            }
+            for (int i = 0; i < Np; ++i) {
+                double suS = 0, suN = 0, sgS = 0, sgN = 0;
+                for (int j = 0; j < Np; ++j) {
+                    suS += lL[j] * u[IDX4(i, j, e, c)];
+                    suN += lR[j] * u[IDX4(i, j, e, c)];
+                    sgS += lL[j] * gy[IDX4(i, j, e, c)];
+                    sgN += lR[j] * gy[IDX4(i, j, e, c)];
+                }
+                uS[BIDX(i, e, c)] = suS; uN[BIDX(i, e, c)] = suN;
+                gyS[BIDX(i, e, c)] = sgS; gyN[BIDX(i, e, c)] = sgN;
+            }
+        }
+    }
+#define BR0X(eL, eR, q, c)                                                                     \
+    br0_avg(gxE[BIDX(q, eL, c)], gxW[BIDX(q, eR, c)], uE[BIDX(q, eL, c)], uW[BIDX(q, eR, c)], \
+            br0_tau(eps[eL], eps[eR], h->dx[(eL) % nx], h->dx[(eR) % nx], p))
+#define BR0Y(eB, eT, q, c)                                                                     \
+    br0_avg(gyN[BIDX(q, eB, c)], gyS[BIDX(q, eT, c)], uN[BIDX(q, eB, c)], uS[BIDX(q, eT, c)], \
+            br0_tau(eps[eB], eps[eT], h->dy[(eB) / nx], h->dy[(eT) / nx], p))
+    for (int jy = 0; jy < ny; ++jy)
+        for (int jx = 0; jx < nx - 1; ++jx) {
+            int eL = jy * nx + jx, eR = eL + 1;
+            for (int q = 0; q < Np; ++q)
+                for (int c = 0; c < NEQ; ++c) {
+                    double gh = BR0X(eL, eR, q, c);
+                    ghE[BIDX(q, eL, c)] = gh;
+                    ghW[BIDX(q, eR, c)] = gh;
+                }
+        }
+    for (int jy = 0; jy < ny; ++jy) {
+        int eL = jy * nx, eR = jy * nx + nx - 1;
+        for (int q = 0; q < Np; ++q)
+            for (int c = 0; c < NEQ; ++c) {
+                if (h->pb.bc_x == ORC_BC_PERIODIC) {
+                    double gh = BR0X(eR, eL, q, c);
+                    ghE[BIDX(q, eR, c)] = gh;
+                    ghW[BIDX(q, eL, c)] = gh;
+                } else {
+                    ghW[BIDX(q, eL, c)] = gxW[BIDX(q, eL, c)];
+                    ghE[BIDX(q, eR, c)] = gxE[BIDX(q, eR, c)];
+                }
+            }
+    }
+    for (int jy = 0; jy < ny - 1; ++jy)
+        for (int jx = 0; jx < nx; ++jx) {
+            int eB = jy * nx + jx, eT = eB + nx;
+            for (int q = 0; q < Np; ++q)
+                for (int c = 0; c < NEQ; ++c) {
+                    double gh = BR0Y(eB, eT, q, c);
+                    ghN[BIDX(q, eB, c)] = gh;
+                    ghS[BIDX(q, eT, c)] = gh;
+                }
+        }
+    for (int jx = 0; jx < nx; ++jx) {
+        int eB = jx, eT = (ny - 1) * nx + jx;
+        for (int q = 0; q < Np; ++q)
+            for (int c = 0; c < NEQ; ++c) {
+                if (h->pb.bc_y == ORC_BC_PERIODIC) {
+                    double gh = BR0Y(eT, eB, q, c);
+                    ghN[BIDX(q, eT, c)] = gh;
+                    ghS[BIDX(q, eB, c)] = gh;
+                } else {
+                    ghS[BIDX(q, eB, c)] = gyS[BIDX(q, eB, c)];
+                    ghN[BIDX(q, eT, c)] = gyN[BIDX(q, eT, c)];
+                }
+            }
+    }
+#undef BR0X
+#undef BR0Y
+    /* FR lift (:353-396) */
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int e = 0; e < Nel; ++e) {
+        int jx = e % nx, jy = e / nx;
+        double Jx = h->Jx[jx], Jy = h->Jy[jy];
+        for (int c = 0; c < NEQ; ++c) {
+            for (int j = 0; j < Np; ++j) {
+                double gLf = 0, gRf = 0;
+                for (int i = 0; i < Np; ++i) {
+                    gLf += lL[i] * gx[IDX4(i, j, e, c)];
+                    gRf += lR[i] * gx[IDX4(i, j, e, c)];
+                }
+                for (int i = 0; i < Np; ++i) {
+                    double vol = 0.0;
+                    for (int k = 0; k < Np; ++k) vol += D[i + Np * k] * gx[IDX4(k, j, e, c)];
+                    double corr = (ghW[BIDX(j, e, c)] - gLf) * gLv[i] + (ghE[BIDX(j, e, c)] - gRf) * gRv[i];
+                    du[IDX4(i, j, e, c)] += (vol + corr) / Jx;
+                }
+            }
+            for (int i = 0; i < Np; ++i) {
+                double gSf = 0, gNf = 0;
+                for (int j = 0; j < Np; ++j) {
+                    gSf += lL[j] * gy[IDX4(i, j, e, c)];
+                    gNf += lR[j] * gy[IDX4(i, j, e, c)];
+                }
+                for (int j = 0; j < Np; ++j) {
+                    double vol = 0.0;
+                    for (int k = 0; k < Np; ++k) vol += D[j + Np * k] * gy[IDX4(i, k, e, c)];
+                    double corr = (ghS[BIDX(i, e, c)] - gSf) * gLv[j] + (ghN[BIDX(i, e, c)] - gNf) * gRv[j];
+                    du[IDX4(i, j, e, c)] += (vol + corr) / Jy;
+                }
+            }
+        }
+    }
+}
+
+/* apply_dissipation_local_DD_2d! (PerssonAV2D.jl:135-194) with eps given */
+static void apply_dd(orc_ctx* h, const double* u, const double* eps, double* du) {
+    const int Np = h->Np, Nel = h->Nel, nx = h->pb.nx;
+    const double* D = h->D;
+#pragma omp parallel num_threads(g_threads)
+    {
+        double* v = (double*)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+        for (int e = 0; e < Nel; ++e) {
+            double ee = eps[e];
+            if (ee <= 0.0) continue;
+            int jx = e % nx, jy = e / nx;
+            double Jx = h->Jx[jx], Jy = h->Jy[jy];
+            double sx = ee / (Jx * Jx), sy = ee / (Jy * Jy);
+            for (int c = 0; c < NEQ; ++c) {
+                for (int j = 0; j < Np; ++j) {
+                    for (int i = 0; i < Np; ++i) {
+                        double s = 0.0;
+                        for (int k = 0; k < Np; ++k) s += D[i + Np * k] * u[IDX4(k, j, e, c)];
+                        v[i] = s;
+                    }
+                    for (int i = 0; i < Np; ++i) {
+                        double s = 0.0;
+                        for (int k = 0; k < Np; ++k) s += D[i + Np * k] * v[k];
+                        du[IDX4(i, j, e, c)] += sx * s;
+                    }
+                }
+                for (int i = 0; i < Np; ++i) {
+                    for (int j = 0; j < Np; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < Np; ++k) s += D[j + Np * k] * u[IDX4(i, k, e, c)];
+                        v[j] = s;
+                    }
+                    for (int j = 0; j < Np; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < Np; ++k) s += D[j + Np * k] * v[k];
+                        du[IDX4(i, j, e, c)] += sy * s;
+                    }
+                }
+            }
+        }
+        free(v);
+    }
+}
+
+void orc_apply_br0_eps(orc_ctx* h, const double* u, const double* eps, double* du) { apply_br0(h, u, eps, du); }
+void orc_apply_dd_eps(orc_ctx* h, const double* u, const double* eps, double* du) { apply_dd(h, u, eps, du); }
+
+/* ---------------------------------------------------------------- time stepping */
+
+/* max_wave_speed (Euler2D.jl:92-104) */
+double orc_max_wave_speed(orc_ctx* h, const double* u) {
+    const int Np = h->Np, Nel = h->Nel;
+    const double g = h->pb.gamma;
+    double m = 0.0;
+    for (int e = 0; e < Nel; ++e)
+        for (int j = 0; j < Np; ++j)
+            for (int i = 0; i < Np; ++i) {
+                double U[4];
+                for (int c = 0; c < 4; ++c) U[c] = u[IDX4(i, j, e, c)];
+                double rho = jmax(U[0], EPS);
+                double uu = U[1] / rho, vv = U[2] / rho;
+                double c = sound_speed(g, U);
+                m = jmax(jmax(m, fabs(uu) + c), fabs(vv) + c);
+            }
+    return jmax(m, EPS);
+}
+
+/* compute_dt 2D (SSP_RK3.jl:126-133) */
+double orc_compute_dt(orc_ctx* h, const double* u, double cfl) {
+    double lam = orc_max_wave_speed(h, u);
+    lam = jmax(lam, EPS);
+    double hmin = h->met[19][0];
+    for (int e = 1; e < h->Nel; ++e) hmin = jmin(hmin, h->met[19][e]);
+    return cfl * hmin / ((double)(2 * h->pb.p + 1) * lam);
+}
+
+static int has_nonfinite(const double* a, size_t n) {
+    for (size_t k = 0; k < n; ++k)
+        if (!isfinite(a[k])) return 1;
+    return 0;
+}
+
+/* ssp_rk3_step! 2D (SSP_RK3.jl:135-164) */
+int orc_rk3_step(orc_ctx* h, double* u, double* t, double dt) {
+    size_t ns = (size_t)h->Np * h->Np * h->Nel * NEQ;
+    double *du = h->du, *u0 = h->u0, *u1 = h->u1, *u2 = h->u2;
+    memcpy(u0, u, ns * sizeof(double));
+    double t0 = *t;
+    orc_residual(h, u, *t, du);
+    if (has_nonfinite(du, ns)) return ORC_DIVERGED;
+    for (size_t k = 0; k < ns; ++k) u[k] = u0[k] + dt * du[k];
+    *t = t0 + dt;
+    orc_residual(h, u, *t, du);
+    if (has_nonfinite(du, ns)) return ORC_DIVERGED;
+    memcpy(u1, u, ns * sizeof(double));
+    const double a34 = 3.0 / 4.0, a14 = 1.0 / 4.0, a13 = 1.0 / 3.0, a23 = 2.0 / 3.0;
+    for (size_t k = 0; k < ns; ++k) u[k] = a34 * u0[k] + a14 * u1[k] + a14 * dt * du[k];
+    *t = t0 + dt / 2.0;
+    orc_residual(h, u, *t, du);
+    if (has_nonfinite(du, ns)) return ORC_DIVERGED;
+    memcpy(u2, u, ns * sizeof(double));
+    for (size_t k = 0; k < ns; ++k) u[k] = a13 * u0[k] + a23 * u2[k] + a23 * dt * du[k];
+    *t = t0 + dt;
+    if (has_nonfinite(u, ns)) return ORC_DIVERGED;
+    return ORC_OK;   /* post_step! is a no-op for every shipped method (Interfaces.jl:90-92) */
+}
+
+/* ssp_rk3! 2D (SSP_RK3.jl:166-211) */
+int orc_run(orc_ctx* h, double* u, double* t, double t_final, double cfl, double fixed_dt,
+            int64_t max_steps, int64_t* n_steps) {
+    int64_t n = 0;
+    while (*t < t_final - 10 * EPS) {
+        double step_dt = fixed_dt > 0 ? fixed_dt : orc_compute_dt(h, u, cfl);
+        if (*t + step_dt > t_final) step_dt = t_final - *t;
+        int st = orc_rk3_step(h, u, t, step_dt);
+        n += 1;
+        if (st != ORC_OK) { *n_steps = n; return st; }
+        if (n >= max_steps) { *n_steps = n; return ORC_MAX_STEPS; }
+    }
+    *n_steps = n;
+    return ORC_OK;
+}

@@ -1,0 +1,85 @@
+/*
+ * frf_oracle.h -- CPU restatement of FRForge's 2D FR residual and SSP-RK3 time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product path is
+ * compared against; it is never linked into, called by or shipped with the product
+ * (frforge_amd/libfrforge_hip.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.
+ *
+ * Layouts follow the reference's Julia column-major arrays (first index fastest):
+ *   u[i,j,e,c]       -> u[i + Np*(j + Np*(e + Nel*c))]          (SolutionState2D.jl:9-18)
+ *   face (Np,Neq,Nel) -> f[q + Np*(c + 4*e)]                    (ResidualWorkspace.jl:49-57)
+ *   vol metrics (Np,Np,Nel), face metrics (Np,Nel)             (MeshMetrics2D.jl:15-39)
+ * Element index e = jy*nx + jx (0-based; Mesh2D.jl:100).
+ *
+ * Parity status of each function is listed in the header comment of frf_oracle.c.
+ */
+#ifndef FRF_ORACLE_H
+#define FRF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_BC_PERIODIC = 0, ORC_BC_TRANSMISSIVE = 1 };
+enum { ORC_FLUX_RUSANOV = 0, ORC_FLUX_HLLC = 1 };
+enum { ORC_CAPT_NULL = 0, ORC_CAPT_PERSSON_BR0 = 1, ORC_CAPT_PERSSON_DD = 2 };
+enum { ORC_OK = 0, ORC_DIVERGED = 1, ORC_MAX_STEPS = 2 };
+
+typedef struct {
+    int32_t p, nx, ny;
+    int32_t bc_x, bc_y;       /* left/right share bc_x, bottom/top share bc_y */
+    int32_t flux;
+    int32_t capturing;
+    double gamma, kappa, s0_factor, eps_floor, c_av;
+} orc_problem;
+
+typedef struct orc_ctx orc_ctx;
+
+/* Reference-element operators (Points.jl, Correction.jl, Operators.jl:85-100).
+ * Arrays sized Np (D and V: Np*Np column-major). GL nodes by Newton on P_Np. */
+int orc_build_operators(int p, double* xi, double* w, double* D, double* lL, double* lR,
+                        double* gLxi, double* gRxi, double* gL, double* gR, double* V);
+
+/* Context owns metrics (full reference arrays) and residual workspace.
+ * Operators and vertices are inputs so that every implementation under test can be
+ * fed identical bits.  xv: nx+1 vertices, yv: ny+1 vertices. */
+orc_ctx* orc_create(const orc_problem* pb, const double* xi, const double* w, const double* D,
+                    const double* lL, const double* lR, const double* gLxi, const double* gRxi,
+                    const double* V, const double* xv, const double* yv);
+void orc_destroy(orc_ctx* h);
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* Metric arrays as built by build_mesh_metrics (affine branch). which: 0..6 = x,y,x_xi,x_eta,
+ * y_xi,y_eta,J (Np*Np*Nel); 7..18 = nx_W,ny_W,sJ_W,nx_E,ny_E,sJ_E,nx_S,ny_S,sJ_S,nx_N,ny_N,sJ_N
+ * (Np*Nel); 19 = h_char (Nel). Returns pointer into the context. */
+const double* orc_metric(orc_ctx* h, int which);
+
+/* residual!(du, state, eq, method) -- Residual2D.jl:131-416 */
+void orc_residual(orc_ctx* h, const double* u, double t, double* du);
+/* sense! (PerssonAV2D.jl:20-59): sigma[Nel] */
+void orc_sense(orc_ctx* h, const double* u, double* sigma);
+/* element_viscosities_2d (PerssonAV2D.jl:89-109): eps[Nel] */
+void orc_element_viscosities(orc_ctx* h, const double* u, const double* sigma, double* eps);
+/* BR0 (PerssonAV2D.jl:203-398) with caller-supplied element viscosities: du += lift */
+void orc_apply_br0_eps(orc_ctx* h, const double* u, const double* eps, double* du);
+/* Element-local DD AV (PerssonAV2D.jl:135-194) with caller-supplied viscosities */
+void orc_apply_dd_eps(orc_ctx* h, const double* u, const double* eps, double* du);
+/* max_wave_speed (Euler2D.jl:92-104) and compute_dt (SSP_RK3.jl:126-133) */
+double orc_max_wave_speed(orc_ctx* h, const double* u);
+double orc_compute_dt(orc_ctx* h, const double* u, double cfl);
+/* ssp_rk3_step! (SSP_RK3.jl:135-164): returns ORC_OK / ORC_DIVERGED; *t updated as reference */
+int orc_rk3_step(orc_ctx* h, double* u, double* t, double dt);
+/* ssp_rk3! (SSP_RK3.jl:166-211). fixed_dt <= 0 selects CFL dt. */
+int orc_run(orc_ctx* h, double* u, double* t, double t_final, double cfl, double fixed_dt,
+            int64_t max_steps, int64_t* n_steps);
+/* Point physics, for unit tests: Rusanov (Euler2D.jl:114-151) and HLLC (HLLC.jl:235-305). */
+void orc_flux_n(int kind, double gamma, const double* uL, const double* uR, double nx, double ny,
+                double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""Throughput benchmark of the 2D FR residual + SSP-RK3 time step on one MI355X.
+
+Workload (the north-star configuration): 2D Euler four-quadrant Riemann problem on [0,1]^2 with
+transmissive sides, 1024 x 1024 quads, p = 3 Gauss-Legendre points, Rusanov flux, Persson sensor
++ BR0 artificial viscosity (c_av = 0.1), SSP-RK3 with a fixed step (CFL 0.08 from the initial
+state).  Quadrant states (rho, u, v, p) are drawn from SplitMix64 (seed 2026): rho~U[0.5,2],
+u,v~U[-1,1], p~U[0.1,1.5].
+
+A step is one SSP-RK3 step (three residual evaluations + stage updates) over the whole mesh with
+the state already resident in HBM.  value = DOF-stage-updates/s of the FP32-mixed form, where
+DOF = Nel (p+1)^2 4 and each step is three stage updates.  The FP64 plain form and the
+NullCapturing variants are timed in the same run and reported beside it, with the density L1
+difference of the mixed form against the FP64 form after the same steps.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+SEED = 2026
+NX = NY = 1024
+P = 3
+CFL = 0.08
+
+# HBM bytes per stage-kernel launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (see profiles/); None until measured for the current kernel.
+MEASURED_TRAFFIC = None
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--p", type=int, default=P)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null")
+    ap.add_argument("--profile-only", default="", help="time only this form:capturing (for rocprof)")
+    return ap.parse_args()
+
+
+def build_state(nx, p, precision, capturing):
+    from frforge_amd import (Euler2D, Mesh2D, PerssonAVMethod, NullCapturing, SchemeConfig,
+                             TransmissiveBC, allocate_state, build_operators, set_initial_condition)
+    from frforge_amd.cases import riemann2d_ic, seeded_quadrant_states
+    eq = Euler2D(1.4)
+    ops = build_operators(p)
+    T = TransmissiveBC
+    mesh = Mesh2D(0.0, 1.0, 0.0, 1.0, nx, nx, left_bc=T(), right_bc=T(), bottom_bc=T(), top_bc=T())
+    st = allocate_state(mesh, ops, scheme=SchemeConfig(flux="rusanov"), precision=precision)
+    cfg = seeded_quadrant_states(SEED)
+    set_initial_condition(st, lambda x, y: riemann2d_ic(eq, x, y, config=cfg))
+    method = PerssonAVMethod.create(c_av=0.1) if capturing == "persson_br0" else NullCapturing()
+    return eq, st, method, cfg
+
+
+def time_form(nx, p, precision, capturing, steps, warmup, dt):
+    import torch
+    from frforge_amd.solver import _device, _push
+    eq, st, method, _ = build_state(nx, p, precision, capturing)
+    dev = _device(st, eq, method)
+    _push(st, dev)
+    if dt is None:
+        dt = dev.compute_dt(CFL)
+    if warmup > 0:
+        assert dev.advance(dt, warmup) == "ok"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    status = dev.advance(dt, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert status == "ok", f"{precision}/{capturing} diverged"
+    prof = dev.profile_steps(dt, min(steps, 10))
+    u = np.empty(st.u.shape, order="F")
+    dev.download(u)
+    dev.close()
+    return {"wall_s": wall, "dt": dt, "profile": prof, "u": u, "w": st.ops.w, "J": st.metrics.J}
+
+
+def run_cpu_baseline(nx, p, dt, capturing="persson_br0"):
+    from oracle import oracle as O
+    eq, st, method, _ = build_state(nx, p, "fp64", capturing)
+    ops = st.ops
+    d = dict(xi=ops.xi, w=ops.w, D=ops.D, lL=ops.ell_L, lR=ops.ell_R, gLxi=ops.gL_xi,
+             gRxi=ops.gR_xi, V=ops.V_legendre)
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(ncpu, 16))
+    O.set_threads(threads)
+    orc = O.Oracle(p, nx, nx, st.mesh.x_vertices, st.mesh.y_vertices, d, bc_x="transmissive",
+                   bc_y="transmissive", flux="rusanov", capturing=capturing, c_av=0.1)
+    u = st.u
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        status, u, _ = orc.rk3_step(u, 0.0, dt)
+        n += 1
+        if time.perf_counter() - t0 > 10.0 or n >= 3:
+            break
+    el = time.perf_counter() - t0
+    dof = nx * nx * (p + 1) ** 2 * 4
+    return {"value": dof * 3 * n / el, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
+            "sample": f"{n} SSP-RK3 step(s) of the same {nx}x{nx} p={p} {capturing} FP64 workload "
+                      f"({el:.1f} s, oracle C restatement, OpenMP {threads} threads)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU domain decomposition is not part of this build)", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    nx, p = a.nx, a.p
+    nel = nx * nx
+    dof = nel * (p + 1) ** 2 * 4
+    S = dof * 8                                  # bytes of one FP64 state array
+    forms = [f.split(":") for f in (a.profile_only or a.forms).split(",")]
+    dt = None
+    res = {}
+    for prec, capt in forms:
+        r = time_form(nx, p, prec, capt, a.steps, a.warmup, dt)
+        dt = r["dt"]
+        res[(prec, capt)] = r
+        print(f"# {prec:10s} {capt:12s} {r['wall_s'] / a.steps * 1e3:8.3f} ms/step  "
+              f"stage kernels {sum(r['profile']['stage_ms']) / sum(r['profile']['stage_launches']) * 1e3:8.1f} us/launch",
+              file=sys.stderr, flush=True)
+    if a.profile_only:
+        return 0
+
+    def summary(key):
+        r = res[key]
+        pr = r["profile"]
+        n_launch = sum(pr["stage_launches"])
+        stage_avg_s = sum(pr["stage_ms"]) / n_launch / 1e3
+        bytes_per_launch = 8.0 * S / 3.0       # mean algorithmic bytes per stage launch
+        achieved = bytes_per_launch / stage_avg_s / 1e9
+        return {"ms_per_step": r["wall_s"] / a.steps * 1e3,
+                "dof_stage_per_s": dof * 3 * a.steps / r["wall_s"],
+                "stage_kernel_us": stage_avg_s * 1e6,
+                "stage_kernel_us_by_stage": [pr["stage_ms"][k] / max(pr["stage_launches"][k], 1) * 1e3 for k in range(3)],
+                "elem_kernel_us": pr["elem_ms"] / max(pr["elem_launches"], 1) * 1e3,
+                "stage_kernel_GBs": achieved}
+
+    head = ("fp32_mixed", "persson_br0")
+    base = ("fp64", "persson_br0")
+    hs, bs = summary(head), summary(base)
+
+    def l1_density(k1, k2):
+        u1, u2 = res[k1]["u"], res[k2]["u"]
+        w, J = res[k1]["w"], res[k1]["J"]
+        W = w[:, None, None] * w[None, :, None] * np.abs(J)
+        d = np.abs(u1[:, :, :, 0] - u2[:, :, :, 0])
+        return float(np.sum(W * d)), float(np.max(d))
+
+    l1, linf = l1_density(head, base)
+    extra = {}
+    if ("fp32_mixed", "null") in res and ("fp64", "null") in res:
+        extra["null_fp32_mixed"] = summary(("fp32_mixed", "null"))
+        extra["null_fp64"] = summary(("fp64", "null"))
+        extra["null_density_l1_mixed_vs_fp64"], _ = l1_density(("fp32_mixed", "null"), ("fp64", "null"))
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = run_cpu_baseline(min(nx, 1024), p, dt)
+
+    line = {
+        "metric": "DOF-stage-updates/s + %HBM roofline, FP32-mixed vs FP64 plain at fixed L1 err",
+        "value": hs["dof_stage_per_s"],
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": hs["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 arithmetic, f64 state",
+        "data": "synthetic (seeded four-quadrant Riemann initial state)",
+        "config": {"workload": f"2D Euler four-quadrant Riemann, {nx}x{nx} quads, p={p} GL, Rusanov, "
+                               "Persson+BR0 AV, SSP-RK3 fixed dt, transmissive",
+                   "nx": nx, "ny": nx, "p": p, "dof": dof, "dt": dt, "seed": SEED,
+                   "form": "fp32_mixed", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": hs["stage_kernel_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": hs["stage_kernel_GBs"] / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC,
+                     "kernel": "stage_kernel (fused residual + SSP-RK3 stage update)",
+                     "algorithmic_bytes_per_launch": 8.0 * S / 3.0},
+        "cpu_baseline": cpu,
+        "fp32_mixed": hs,
+        "fp64_plain": bs,
+        "speedup_mixed_over_fp64": bs["ms_per_step"] / hs["ms_per_step"],
+        "density_l1_mixed_vs_fp64": l1,
+        "density_linf_mixed_vs_fp64": linf,
+        **extra,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

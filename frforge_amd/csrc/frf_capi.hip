@@ -1,0 +1,622 @@
+// C ABI of the FRForge 2D FR device path (declared in include/frforge_hip.h).
+//
+// The context owns: three state buffers (u^n, u^(1), u^(2)), element sensor/viscosity arrays,
+// the affine metric tables and the device step-control block.  Stage 3 writes u^(n+1) over u^n,
+// so a step moves exactly the reference's state bytes: stage 1 reads u^n and writes u^(1);
+// stages 2 and 3 read u^(k) and u^n and write one state.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/frforge_hip.h"
+#include "frf_kernels.h"
+#include "frf_launch.h"
+
+using frf::DevCtl;
+using frf::HostOps;
+using frf::kNoFail;
+
+struct frf_ctx {
+    frf_problem pb;
+    int np = 0, nel = 0;
+    size_t nstate = 0;                 // values per state array
+    std::string err;
+    hipStream_t stream = nullptr;
+    double* dU[3] = {nullptr, nullptr, nullptr};
+    double* dSigma = nullptr;
+    double* dEps = nullptr;
+    double* dCtab = nullptr;
+    double* dRtab = nullptr;
+    DevCtl* dCtl = nullptr;
+    DevCtl* hCtl = nullptr;            // pinned host mirror for polling
+    HostOps ops{};
+    bool ops_set = false, geom_set = false, state_set = false;
+    std::vector<double> xi_nodes;     // solution points, for the metric tables
+    double hmin = 0.0;
+    double t = 0.0;
+    int cur = 0;                       // which of dU holds the state as the reference sees it
+    // optional per-launch event timing (frf_profile_steps)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;   // begin/end pairs
+    std::vector<int> prof_kind;        // 0..2 stage 1..3, 3 element pass, 4 step control
+};
+
+static void prof_mark(frf_ctx* c) {
+    if (!c->prof) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->prof_ev.push_back(e);
+}
+static void prof_kind(frf_ctx* c, int kind) {
+    if (c->prof) c->prof_kind.push_back(kind);
+}
+
+static thread_local std::string g_create_err;
+
+static int32_t fail(frf_ctx* c, int32_t code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf; else g_create_err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                      \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess) return fail((c), FRF_E_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+static const double kEps = 2.220446049250313080847e-16;
+
+extern "C" int32_t frf_create(const frf_problem* pb, frf_ctx** out) {
+    if (!pb || !out) return fail(nullptr, FRF_E_ARG, "frf_create: null argument");
+    *out = nullptr;
+    if (pb->p < 1 || pb->p > 5) return fail(nullptr, FRF_E_UNSUPPORTED, "degree p=%d not in 1..5", pb->p);
+    if (pb->nx < 1 || pb->ny < 1) return fail(nullptr, FRF_E_ARG, "nx, ny must be >= 1");
+    if (pb->neq != 4) return fail(nullptr, FRF_E_UNSUPPORTED, "neq=%d: only Euler2D (4) is supported", pb->neq);
+    auto okbc = [](int b) { return b == FRF_BC_PERIODIC || b == FRF_BC_TRANSMISSIVE; };
+    if (!okbc(pb->bc_left) || !okbc(pb->bc_right) || !okbc(pb->bc_bottom) || !okbc(pb->bc_top))
+        return fail(nullptr, FRF_E_UNSUPPORTED, "only periodic and transmissive sides are supported");
+    if ((pb->bc_left == FRF_BC_PERIODIC) != (pb->bc_right == FRF_BC_PERIODIC))
+        return fail(nullptr, FRF_E_ARG, "x-periodic requires both left and right PeriodicBC");
+    if ((pb->bc_bottom == FRF_BC_PERIODIC) != (pb->bc_top == FRF_BC_PERIODIC))
+        return fail(nullptr, FRF_E_ARG, "y-periodic requires both bottom and top PeriodicBC");
+    if (pb->flux != FRF_FLUX_RUSANOV && pb->flux != FRF_FLUX_HLLC)
+        return fail(nullptr, FRF_E_ARG, "unknown flux kind %d", pb->flux);
+    if (pb->precision != FRF_FP64 && pb->precision != FRF_FP32_MIXED)
+        return fail(nullptr, FRF_E_ARG, "unknown precision %d", pb->precision);
+    if (pb->capturing < FRF_CAPT_NULL || pb->capturing > FRF_CAPT_PERSSON_DD)
+        return fail(nullptr, FRF_E_ARG, "unknown capturing kind %d", pb->capturing);
+    if ((int64_t)pb->nx * pb->ny > (int64_t)1 << 28)
+        return fail(nullptr, FRF_E_ARG, "mesh too large");
+
+    frf_ctx* c = new frf_ctx();
+    c->pb = *pb;
+    c->np = pb->p + 1;
+    c->nel = pb->nx * pb->ny;
+    c->nstate = (size_t)c->np * c->np * c->nel * 4;
+    const int ts = frf::tab_stride(c->np);
+    auto cleanup = [&](int32_t code) {
+        g_create_err = c->err;
+        frf_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(c, FRF_E_HIP, "hipStreamCreate failed"));
+    for (int k = 0; k < 3; ++k)
+        if (hipMalloc(&c->dU[k], c->nstate * sizeof(double)) != hipSuccess)
+            return cleanup(fail(c, FRF_E_HIP, "hipMalloc of %zu state bytes failed", c->nstate * 8));
+    if (hipMalloc(&c->dSigma, c->nel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dEps, c->nel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dCtab, (size_t)pb->nx * ts * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dRtab, (size_t)pb->ny * ts * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dCtl, sizeof(DevCtl)) != hipSuccess ||
+        hipHostMalloc(&c->hCtl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(c, FRF_E_HIP, "device allocation failed"));
+    (void)hipMemsetAsync(c->dEps, 0, c->nel * sizeof(double), c->stream);
+    *out = c;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_destroy(frf_ctx* c) {
+    if (!c) return FRF_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 3; ++k) if (c->dU[k]) (void)hipFree(c->dU[k]);
+    if (c->dSigma) (void)hipFree(c->dSigma);
+    if (c->dEps) (void)hipFree(c->dEps);
+    if (c->dCtab) (void)hipFree(c->dCtab);
+    if (c->dRtab) (void)hipFree(c->dRtab);
+    if (c->dCtl) (void)hipFree(c->dCtl);
+    if (c->hCtl) (void)hipHostFree(c->hCtl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FRF_OK;
+}
+
+extern "C" const char* frf_last_error(const frf_ctx* c) {
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" void* frf_stream(frf_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+// Inverse of the Legendre Vandermonde by Gauss-Jordan elimination with partial pivoting.
+// The reference applies V^{-1} through LU solves (PerssonAV2D.jl:45-46); both give V^{-1} U V^{-T}
+// to a few ulps.
+static bool invert(int n, const double* A /* col-major */, double* inv /* row-major a*n+i */) {
+    std::vector<double> M((size_t)n * 2 * n);
+    for (int r = 0; r < n; ++r) {
+        for (int k = 0; k < n; ++k) M[r * 2 * n + k] = A[r + n * k];
+        for (int k = 0; k < n; ++k) M[r * 2 * n + n + k] = (r == k) ? 1.0 : 0.0;
+    }
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(M[r * 2 * n + col]) > std::fabs(M[piv * 2 * n + col])) piv = r;
+        if (M[piv * 2 * n + col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < 2 * n; ++k) std::swap(M[col * 2 * n + k], M[piv * 2 * n + k]);
+        const double d = M[col * 2 * n + col];
+        for (int k = 0; k < 2 * n; ++k) M[col * 2 * n + k] /= d;
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = M[r * 2 * n + col];
+            if (f == 0.0) continue;
+            for (int k = 0; k < 2 * n; ++k) M[r * 2 * n + k] -= f * M[col * 2 * n + k];
+        }
+    }
+    for (int a = 0; a < n; ++a)
+        for (int i = 0; i < n; ++i) inv[a * n + i] = M[a * 2 * n + n + i];
+    return true;
+}
+
+extern "C" int32_t frf_set_operators(frf_ctx* c, const double* xi, const double* w, const double* D,
+                                     const double* lL, const double* lR, const double* gLxi,
+                                     const double* gRxi, const double* V) {
+    if (!c) return FRF_E_ARG;
+    if (!xi || !D || !lL || !lR || !gLxi || !gRxi || !V) return fail(c, FRF_E_ARG, "null operator array");
+    (void)w;
+    const int n = c->np;
+    HostOps h{};
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) h.D[i * n + k] = D[i + n * k];
+    for (int i = 0; i < n; ++i) {
+        h.lL[i] = lL[i];
+        h.lR[i] = lR[i];
+        h.gLxi[i] = gLxi[i];
+        h.gRxi[i] = gRxi[i];
+    }
+    if (!invert(n, V, h.Vinv)) return fail(c, FRF_E_ARG, "Legendre Vandermonde is singular");
+    c->ops = h;
+    c->ops_set = true;
+    c->geom_set = false;   // tables depend on D and the Lagrange endpoint values
+    c->xi_nodes.assign(xi, xi + n);
+    return FRF_OK;
+}
+
+// Affine metric tables.  For x = xc(jx) + Jx(jx) xi_i (independent of eta) and
+// y = yc(jy) + Jy(jy) xi_j, the reference's D-derived metrics (MeshMetrics2D.jl:96-122) are
+//   x_xi[i]     = sum_k D[i,k] X[k]     (per column jx)
+//   x_eta[i,j]  = sum_k D[j,k] X[i]     (per column jx; roundoff-sized, not zero)
+//   y_xi[i,j]   = sum_k D[i,k] Y[j]     (per row jy;   roundoff-sized, not zero)
+//   y_eta[j]    = sum_k D[j,k] Y[k]     (per row jy)
+// and their Lagrange extrapolations to the faces (:124-184).  Computed here in the reference's
+// summation order without fused multiply-adds, so the device reproduces the reference's
+// metric values and normals exactly.
+extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* yv) {
+#pragma clang fp contract(off)
+    if (!c) return FRF_E_ARG;
+    if (!c->ops_set) return fail(c, FRF_E_STATE, "frf_set_operators must be called first");
+    if (!xv || !yv) return fail(c, FRF_E_ARG, "null vertex array");
+    const int n = c->np, nx = c->pb.nx, ny = c->pb.ny;
+    for (int k = 0; k < nx; ++k)
+        if (!(xv[k + 1] > xv[k])) return fail(c, FRF_E_ARG, "x vertices must increase");
+    for (int k = 0; k < ny; ++k)
+        if (!(yv[k + 1] > yv[k])) return fail(c, FRF_E_ARG, "y vertices must increase");
+    const HostOps& h = c->ops;
+    const double* xi = c->xi_nodes.data();
+    const int ts = frf::tab_stride(n);
+    const int XI = 0, CROSS = n, FW = n + n * n, FE = FW + n, FS = FE + n, FN = FS + n, JAC = FN + n,
+              DELTA = JAC + 1;
+    std::vector<double> ct((size_t)nx * ts, 0.0), rt((size_t)ny * ts, 0.0);
+    // columns (x direction): X[i] = (xL + xR)/2 + Jx * xi_i   (MeshMetrics2D.jl:49-58)
+    for (int k = 0; k < nx; ++k) {
+        double* T = &ct[(size_t)k * ts];
+        const double xL = xv[k], xR = xv[k + 1];
+        const double dx = xR - xL, Jx = dx / 2.0;
+        double X[frf::kMaxNp];
+        for (int i = 0; i < n; ++i) X[i] = (xL + xR) / 2.0 + Jx * xi[i];
+        for (int i = 0; i < n; ++i) {           // x_xi[i] = sum_q D[i,q] X[q]
+            double s = 0.0;
+            for (int q = 0; q < n; ++q) s += h.D[i * n + q] * X[q];
+            T[XI + i] = s;
+        }
+        for (int j = 0; j < n; ++j)             // x_eta[i,j] = sum_q D[j,q] X[i]
+            for (int i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (int q = 0; q < n; ++q) s += h.D[j * n + q] * X[i];
+                T[CROSS + j * n + i] = s;
+            }
+        for (int j = 0; j < n; ++j) {           // x_eta at W/E faces (row j)
+            double a = 0.0, b = 0.0;
+            for (int i = 0; i < n; ++i) {
+                a += h.lL[i] * T[CROSS + j * n + i];
+                b += h.lR[i] * T[CROSS + j * n + i];
+            }
+            T[FW + j] = a;
+            T[FE + j] = b;
+        }
+        for (int i = 0; i < n; ++i) {           // x_xi at S/N faces (column i)
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < n; ++j) {
+                a += h.lL[j] * T[XI + i];
+                b += h.lR[j] * T[XI + i];
+            }
+            T[FS + i] = a;
+            T[FN + i] = b;
+        }
+        T[JAC] = Jx;
+        T[DELTA] = dx;
+    }
+    // rows (y direction): Y[j] = (yB + yT)/2 + Jy * xi_j
+    for (int k = 0; k < ny; ++k) {
+        double* T = &rt[(size_t)k * ts];
+        const double yB = yv[k], yT = yv[k + 1];
+        const double dy = yT - yB, Jy = dy / 2.0;
+        double Y[frf::kMaxNp];
+        for (int j = 0; j < n; ++j) Y[j] = (yB + yT) / 2.0 + Jy * xi[j];
+        for (int j = 0; j < n; ++j) {           // y_eta[j] = sum_q D[j,q] Y[q]
+            double s = 0.0;
+            for (int q = 0; q < n; ++q) s += h.D[j * n + q] * Y[q];
+            T[XI + j] = s;
+        }
+        for (int j = 0; j < n; ++j)             // y_xi[i,j] = sum_q D[i,q] Y[j]
+            for (int i = 0; i < n; ++i) {
+                double s = 0.0;
+                for (int q = 0; q < n; ++q) s += h.D[i * n + q] * Y[j];
+                T[CROSS + j * n + i] = s;
+            }
+        for (int j = 0; j < n; ++j) {           // y_eta at W/E faces (row j)
+            double a = 0.0, b = 0.0;
+            for (int i = 0; i < n; ++i) {
+                a += h.lL[i] * T[XI + j];
+                b += h.lR[i] * T[XI + j];
+            }
+            T[FW + j] = a;
+            T[FE + j] = b;
+        }
+        for (int i = 0; i < n; ++i) {           // y_xi at S/N faces (column i)
+            double a = 0.0, b = 0.0;
+            for (int j = 0; j < n; ++j) {
+                a += h.lL[j] * T[CROSS + j * n + i];
+                b += h.lR[j] * T[CROSS + j * n + i];
+            }
+            T[FS + i] = a;
+            T[FN + i] = b;
+        }
+        T[JAC] = Jy;
+        T[DELTA] = dy;
+    }
+    double hmin = INFINITY;
+    for (int k = 0; k < nx; ++k) hmin = std::fmin(hmin, ct[(size_t)k * ts + DELTA]);
+    for (int k = 0; k < ny; ++k) hmin = std::fmin(hmin, rt[(size_t)k * ts + DELTA]);
+    c->hmin = hmin;
+    HIPCHK(c, hipMemcpyAsync(c->dCtab, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dRtab, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->geom_set = true;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_upload_state(frf_ctx* c, const double* u, int64_t len, double t) {
+    if (!c) return FRF_E_ARG;
+    if (!u || len != (int64_t)c->nstate)
+        return fail(c, FRF_E_ARG, "state length %lld != %zu", (long long)len, c->nstate);
+    HIPCHK(c, hipMemcpyAsync(c->dU[0], u, c->nstate * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->t = t;
+    c->state_set = true;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_download_state(frf_ctx* c, double* u, int64_t len, double* t) {
+    if (!c) return FRF_E_ARG;
+    if (!c->state_set) return fail(c, FRF_E_STATE, "no state uploaded");
+    if (!u || len != (int64_t)c->nstate)
+        return fail(c, FRF_E_ARG, "state length %lld != %zu", (long long)len, c->nstate);
+    HIPCHK(c, hipMemcpyAsync(u, c->dU[c->cur], c->nstate * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t) *t = c->t;
+    return FRF_OK;
+}
+
+static int32_t ready(frf_ctx* c) {
+    if (!c) return FRF_E_ARG;
+    if (!c->ops_set) return fail(c, FRF_E_STATE, "operators not set");
+    if (!c->geom_set) return fail(c, FRF_E_STATE, "geometry not set");
+    if (!c->state_set) return fail(c, FRF_E_STATE, "no state uploaded");
+    return FRF_OK;
+}
+
+static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double* u0, double* uout, int stage) {
+    frf::StageParams P{};
+    P.uin = uin;
+    P.u0 = u0;
+    P.uout = uout;
+    P.eps = c->dEps;
+    P.ctab = c->dCtab;
+    P.rtab = c->dRtab;
+    P.ctl = c->dCtl;
+    P.gamma = c->pb.gamma;
+    P.nx = c->pb.nx;
+    P.ny = c->pb.ny;
+    P.nel = c->nel;
+    P.bcx = c->pb.bc_left == FRF_BC_PERIODIC ? 0 : 1;
+    P.bcy = c->pb.bc_bottom == FRF_BC_PERIODIC ? 0 : 1;
+    P.stage = stage;
+    P.p = c->pb.p;
+    return P;
+}
+
+static frf::ElemParams elem_params(frf_ctx* c, const double* u, int want_sigma) {
+    frf::ElemParams E{};
+    E.u = u;
+    E.sigma = c->dSigma;
+    E.eps = c->dEps;
+    E.ctab = c->dCtab;
+    E.rtab = c->dRtab;
+    E.ctl = c->dCtl;
+    E.gamma = c->pb.gamma;
+    E.kappa = c->pb.kappa;
+    E.s0 = c->pb.s0_factor * std::log10((double)(c->pb.p > 1 ? c->pb.p : 1));
+    E.eps_floor = c->pb.eps_floor;
+    E.c_av = c->pb.c_av;
+    E.nx = c->pb.nx;
+    E.ny = c->pb.ny;
+    E.nel = c->nel;
+    E.p = c->pb.p;
+    E.want_sigma = want_sigma;
+    return E;
+}
+
+// Make u^n the device's current buffer 0 (after a diverged step the observable state may sit
+// in another buffer).
+static int32_t settle(frf_ctx* c) {
+    if (c->cur != 0) {
+        HIPCHK(c, hipMemcpyAsync(c->dU[0], c->dU[c->cur], c->nstate * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+        c->cur = 0;
+    }
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_residual(frf_ctx* c, double* du, int64_t len) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if (!du || len != (int64_t)c->nstate)
+        return fail(c, FRF_E_ARG, "du length %lld != %zu", (long long)len, c->nstate);
+    if ((r = settle(c))) return r;
+    const int prec = c->pb.precision, capt = c->pb.capturing;
+    if (capt) {
+        frf::ElemParams E = elem_params(c, c->dU[0], 1);
+        HIPCHK(c, frf::launch_elem(c->np, prec, E, c->ops, 0, 0, c->stream));
+    }
+    frf::StageParams P = stage_params(c, c->dU[0], c->dU[0], c->dU[1], 0);
+    HIPCHK(c, frf::launch_stage(c->np, prec, c->pb.flux, capt, P, c->ops, c->stream));
+    HIPCHK(c, hipMemcpyAsync(du, c->dU[1], c->nstate * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_sense(frf_ctx* c, double* sigma, double* eps) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if ((r = settle(c))) return r;
+    frf::ElemParams E = elem_params(c, c->dU[0], 1);
+    HIPCHK(c, frf::launch_elem(c->np, c->pb.precision, E, c->ops, 0, 0, c->stream));
+    if (sigma) HIPCHK(c, hipMemcpyAsync(sigma, c->dSigma, c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (eps) HIPCHK(c, hipMemcpyAsync(eps, c->dEps, c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FRF_OK;
+}
+
+static void init_ctl(frf_ctx* c, double t_final, double cfl, double fixed_dt, int64_t max_steps) {
+    DevCtl& h = *c->hCtl;
+    std::memset(&h, 0, sizeof h);
+    h.t = c->t;
+    h.t0 = c->t;
+    h.t_final = t_final;
+    h.cfl = cfl;
+    h.fixed_dt = fixed_dt;
+    h.hmin = c->hmin;
+    h.lam_bits = 0ull;
+    h.n_steps = 0;
+    h.max_steps = max_steps;
+    h.status = 0;
+    h.fail_stage = kNoFail;
+    h.done = 0;
+    h.p = c->pb.p;
+}
+
+// One SSP-RK3 step on the device (no host synchronisation): the sequence of launches that a
+// run repeats.  `adaptive` adds the wave-speed pass and the step-control kernel.
+static int32_t enqueue_step(frf_ctx* c, bool with_dt_kernel, bool need_lambda) {
+    const int prec = c->pb.precision, capt = c->pb.capturing, flux = c->pb.flux, np = c->np;
+    double *U0 = c->dU[0], *U1 = c->dU[1], *U2 = c->dU[2];
+#define FRF_TIMED(kind, expr) \
+    do {                      \
+        prof_mark(c);         \
+        HIPCHK(c, expr);      \
+        prof_mark(c);         \
+        prof_kind(c, kind);   \
+    } while (0)
+    if (capt || need_lambda) {
+        frf::ElemParams E = elem_params(c, U0, capt ? 1 : 0);
+        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, need_lambda ? 1 : 0, 1, c->stream));
+    }
+    if (with_dt_kernel) FRF_TIMED(4, frf::launch_dt(c->dCtl, c->stream));
+    frf::StageParams P1 = stage_params(c, U0, U0, U1, 1);
+    FRF_TIMED(0, frf::launch_stage(np, prec, flux, capt, P1, c->ops, c->stream));
+    if (capt) {
+        frf::ElemParams E = elem_params(c, U1, 1);
+        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, 0, 1, c->stream));
+    }
+    frf::StageParams P2 = stage_params(c, U1, U0, U2, 2);
+    FRF_TIMED(1, frf::launch_stage(np, prec, flux, capt, P2, c->ops, c->stream));
+    if (capt) {
+        frf::ElemParams E = elem_params(c, U2, 1);
+        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, 0, 1, c->stream));
+    }
+    frf::StageParams P3 = stage_params(c, U2, U0, U0, 3);
+    FRF_TIMED(2, frf::launch_stage(np, prec, flux, capt, P3, c->ops, c->stream));
+    FRF_TIMED(4, frf::launch_finish(c->dCtl, c->stream));
+#undef FRF_TIMED
+    return FRF_OK;
+}
+
+static int32_t read_ctl(frf_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->hCtl, c->dCtl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FRF_OK;
+}
+
+// Where the reference leaves state.u after a step that ended with fail stage fs.
+static int observable_buffer(int fs) { return fs == 2 ? 1 : (fs == 3 ? 2 : 0); }
+
+extern "C" int32_t frf_rk3_step(frf_ctx* c, double dt, int32_t* status) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if ((r = settle(c))) return r;
+    init_ctl(c, INFINITY, 0.0, dt, 1 << 30);
+    c->hCtl->dt = dt;
+    c->hCtl->t0 = c->t;
+    HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    if ((r = enqueue_step(c, false, false))) return r;
+    if ((r = read_ctl(c))) return r;
+    c->t = c->hCtl->t;
+    const int fs = c->hCtl->fail_stage;
+    c->cur = fs < kNoFail ? observable_buffer(fs) : 0;
+    if (status) *status = fs < kNoFail ? FRF_STATUS_DIVERGED : FRF_STATUS_OK;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_advance(frf_ctx* c, double dt, int64_t n_steps, int32_t* status) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if (!(dt > 0.0) || n_steps < 0) return fail(c, FRF_E_ARG, "dt must be > 0 and n_steps >= 0");
+    if ((r = settle(c))) return r;
+    init_ctl(c, INFINITY, 0.0, dt, n_steps > 0 ? n_steps : 1);
+    HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    for (int64_t s = 0; s < n_steps; ++s)
+        if ((r = enqueue_step(c, true, false))) return r;
+    if ((r = read_ctl(c))) return r;
+    c->t = c->hCtl->t;
+    const int fs = c->hCtl->fail_stage;
+    c->cur = fs < kNoFail ? observable_buffer(fs) : 0;
+    if (status) *status = fs < kNoFail ? FRF_STATUS_DIVERGED : FRF_STATUS_OK;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_run(frf_ctx* c, double t_final, double cfl, double fixed_dt,
+                           int64_t max_steps, frf_run_result* res) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if (!res) return fail(c, FRF_E_ARG, "null result");
+    if (max_steps < 1) return fail(c, FRF_E_ARG, "max_steps must be >= 1");
+    if (!(fixed_dt > 0.0) && !(cfl > 0.0)) return fail(c, FRF_E_ARG, "cfl must be > 0 for CFL steps");
+    if ((r = settle(c))) return r;
+    init_ctl(c, t_final, cfl, fixed_dt > 0.0 ? fixed_dt : 0.0, max_steps);
+    HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    const bool adaptive = !(fixed_dt > 0.0);
+    int64_t batch = 4;
+    while (true) {
+        for (int64_t s = 0; s < batch; ++s)
+            if ((r = enqueue_step(c, true, adaptive))) return r;
+        if ((r = read_ctl(c))) return r;
+        if (c->hCtl->done) break;
+        if (batch < 64) batch *= 2;
+    }
+    const DevCtl& h = *c->hCtl;
+    c->t = h.t;
+    const int fs = h.fail_stage;
+    c->cur = fs < kNoFail ? observable_buffer(fs) : 0;
+    res->status = h.status;
+    res->fail_stage = fs < kNoFail ? fs : 0;
+    res->n_steps = h.n_steps;
+    res->t = h.t;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_compute_dt(frf_ctx* c, double cfl, double* dt) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if (!dt) return fail(c, FRF_E_ARG, "null dt");
+    if ((r = settle(c))) return r;
+    init_ctl(c, INFINITY, cfl, 0.0, 1);
+    HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    frf::ElemParams E = elem_params(c, c->dU[0], 0);
+    HIPCHK(c, frf::launch_elem(c->np, c->pb.precision, E, c->ops, 1, 0, c->stream));
+    if ((r = read_ctl(c))) return r;
+    double lam;
+    std::memcpy(&lam, &c->hCtl->lam_bits, sizeof lam);
+    lam = (lam > kEps || std::isnan(lam)) ? lam : kEps;
+    *dt = cfl * c->hmin / ((double)(2 * c->pb.p + 1) * lam);
+    return FRF_OK;
+}
+
+// Same steps as frf_advance, with a HIP event pair recorded on the context's stream around
+// every kernel launch; reports the summed device time per kernel kind.
+extern "C" int32_t frf_profile_steps(frf_ctx* c, double dt, int64_t n_steps, frf_profile* out) {
+    int32_t r = ready(c);
+    if (r) return r;
+    if (!out || !(dt > 0.0) || n_steps < 1 || n_steps > 4096)
+        return fail(c, FRF_E_ARG, "need dt > 0, 1 <= n_steps <= 4096 and an output struct");
+    if ((r = settle(c))) return r;
+    std::memset(out, 0, sizeof *out);
+    init_ctl(c, INFINITY, 0.0, dt, n_steps);
+    HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    c->prof = true;
+    c->prof_ev.clear();
+    c->prof_kind.clear();
+    for (int64_t s = 0; s < n_steps && r == FRF_OK; ++s) r = enqueue_step(c, true, false);
+    c->prof = false;
+    int32_t r2 = read_ctl(c);
+    for (size_t k = 0; k < c->prof_kind.size() && 2 * k + 1 < c->prof_ev.size(); ++k) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->prof_ev[2 * k], c->prof_ev[2 * k + 1]) != hipSuccess) continue;
+        const int kind = c->prof_kind[k];
+        if (kind <= 2) {
+            out->stage_ms[kind] += ms;
+            out->stage_launches[kind] += 1;
+        } else if (kind == 3) {
+            out->elem_ms += ms;
+            out->elem_launches += 1;
+        } else {
+            out->control_ms += ms;
+            out->control_launches += 1;
+        }
+    }
+    if (c->prof_ev.size() >= 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->prof_ev.front(), c->prof_ev.back()) == hipSuccess) out->span_ms = ms;
+    }
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+    c->prof_kind.clear();
+    if (r) return r;
+    if (r2) return r2;
+    c->t = c->hCtl->t;
+    const int fs = c->hCtl->fail_stage;
+    c->cur = fs < kNoFail ? observable_buffer(fs) : 0;
+    out->status = fs < kNoFail ? FRF_STATUS_DIVERGED : FRF_STATUS_OK;
+    return FRF_OK;
+}

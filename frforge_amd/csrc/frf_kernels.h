@@ -1,0 +1,91 @@
+// Device-side structures shared by the kernels and the host library.
+#pragma once
+#include <cstdint>
+
+namespace frf {
+
+constexpr int kNeq = 4;
+constexpr int kMaxNp = 6;   // p <= 5
+constexpr int kNoFail = 99; // DevCtl::fail_stage when no stage has failed
+
+// Step-control block kept in device memory so a whole run can be captured in a HIP graph:
+// the kernels read dt and the status flags from here instead of from kernel arguments.
+struct DevCtl {
+    double t;                     // state time
+    double t0;                    // time at the start of the current step
+    double dt;                    // current step size
+    double t_final;
+    double cfl;
+    double fixed_dt;              // > 0: fixed step (last one truncated); else CFL step
+    double hmin;                  // min over elements of h_char
+    unsigned long long lam_bits;  // running max of the wave speed (bits of a non-negative double)
+    int64_t n_steps;
+    int64_t max_steps;
+    int32_t status;               // 0 ok, 1 diverged, 2 max_steps
+    int32_t fail_stage;           // 1..3: non-finite du at that stage; 4: non-finite u at step
+                                  // end; kNoFail otherwise (stages report with atomicMin)
+    int32_t done;                 // loop finished (t reached, diverged or max_steps)
+    int32_t flags;                // bit0: non-finite seen in the last residual-only launch
+    int32_t p;
+    int32_t pad;
+};
+
+// Operators, passed by value (kernel-argument segment, read through scalar loads).
+template <int NP>
+struct OpsT {
+    double D[NP * NP];      // D[i*NP + k] = D[i, k]
+    double lL[NP], lR[NP];  // Lagrange basis at xi = -1, +1
+    double gL[NP], gR[NP];  // g_DG derivatives at the SPs (gL_xi, gR_xi)
+    double Vinv[NP * NP];   // inverse Legendre Vandermonde, Vinv[a*NP + i]
+};
+
+// Per-column (jx) and per-row (jy) metric tables of an affine mesh; see frf_capi.hip.
+template <int NP>
+struct TabLayout {
+    static constexpr int XI = 0;                 // x_xi[i] | y_eta[j]
+    static constexpr int CROSS = NP;             // x_eta[j*NP+i] | y_xi[j*NP+i]
+    static constexpr int FW = NP + NP * NP;      // face W (row j): x_eta | y_eta
+    static constexpr int FE = FW + NP;           // face E
+    static constexpr int FS = FE + NP;           // face S (column i): x_xi | y_xi
+    static constexpr int FN = FS + NP;           // face N
+    static constexpr int JAC = FN + NP;          // Jx | Jy
+    static constexpr int DELTA = JAC + 1;        // dx | dy
+    static constexpr int STRIDE = DELTA + 1;
+};
+
+constexpr int tab_stride(int np) { return 5 * np + np * np + 2; }
+static_assert(TabLayout<2>::STRIDE == tab_stride(2) && TabLayout<3>::STRIDE == tab_stride(3) &&
+                  TabLayout<4>::STRIDE == tab_stride(4) && TabLayout<5>::STRIDE == tab_stride(5) &&
+                  TabLayout<6>::STRIDE == tab_stride(6),
+              "host and device metric-table layouts differ");
+
+struct StageParams {
+    const double* uin;    // state the residual is evaluated on
+    const double* u0;     // u^n (stages 2 and 3)
+    double* uout;         // next stage state, or du for a residual-only launch
+    const double* eps;    // element viscosities of uin (capturing methods)
+    const double* ctab;
+    const double* rtab;
+    DevCtl* ctl;
+    double gamma;
+    int nx, ny, nel;
+    int bcx, bcy;         // 0 periodic, 1 transmissive
+    int stage;            // 0 residual only; 1..3 SSP-RK3 stage
+    int nblk;
+    int p;
+};
+
+struct ElemParams {
+    const double* u;
+    double* sigma;        // may be null
+    double* eps;          // may be null
+    const double* ctab;
+    const double* rtab;
+    DevCtl* ctl;
+    double gamma, kappa, s0, eps_floor, c_av;
+    int nx, ny, nel, nblk;
+    int p;
+    int want_sigma;       // compute sigma/eps (capturing) in addition to the wave speed
+};
+
+}  // namespace frf

@@ -1,0 +1,26 @@
+// Host-side launch entry points of the kernels in frf_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "frf_kernels.h"
+
+namespace frf {
+
+// Operators as uploaded by the caller, padded to the largest supported degree.
+struct HostOps {
+    double D[kMaxNp * kMaxNp];     // D[i*Np + k]
+    double lL[kMaxNp], lR[kMaxNp];
+    double gLxi[kMaxNp], gRxi[kMaxNp];
+    double Vinv[kMaxNp * kMaxNp];  // Vinv[a*Np + i]
+};
+
+// prec: 0 = fp64, 1 = fp32-mixed; flux: 0 = Rusanov, 1 = HLLC;
+// capt: 0 = none, 1 = Persson + BR0, 2 = Persson + element-local DD.
+hipError_t launch_stage(int np, int prec, int flux, int capt, const StageParams& P,
+                        const HostOps& h, hipStream_t s);
+hipError_t launch_elem(int np, int prec, const ElemParams& P, const HostOps& h, int accum_lambda,
+                       int check_ctl, hipStream_t s);
+hipError_t launch_dt(DevCtl* c, hipStream_t s);
+hipError_t launch_finish(DevCtl* c, hipStream_t s);
+
+}  // namespace frf

@@ -1,0 +1,160 @@
+// Point physics of 2D compressible Euler for the device kernels, templated on the arithmetic
+// type R (double for the plain form, float for the FP32-mixed form).
+//
+// Follows FRForge src/equations/Euler2D.jl and src/flux/HLLC.jl; every density/pressure floor
+// uses the float64 machine epsilon, as the reference does for its Float64 runs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace frf {
+
+constexpr double kEps64 = 2.220446049250313080847e-16;   // eps(Float64)
+
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rabs(R a) { return a < R(0) ? -a : a; }
+
+__device__ __forceinline__ double rsqrt_(double a) { return sqrt(a); }
+__device__ __forceinline__ float rsqrt_(float a) { return sqrtf(a); }
+__device__ __forceinline__ double rhypot(double a, double b) { return hypot(a, b); }
+__device__ __forceinline__ float rhypot(float a, float b) { return hypotf(a, b); }
+
+// Conserved U = (rho, rho u, rho v, E); primitive helpers with the rho floor
+// (Euler2D.jl:14-19, 35-39).
+template <typename R>
+struct Prim {
+    R rho;   // max(U1, eps)
+    R u, v;  // velocities
+    R p;     // pressure (no floor)
+};
+
+template <typename R>
+__device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
+    Prim<R> q;
+    q.rho = rmax(U[0], R(kEps64));
+    R inv = R(1) / q.rho;
+    q.u = U[1] * inv;
+    q.v = U[2] * inv;
+    q.p = (gamma - R(1)) * (U[3] - R(0.5) * q.rho * (q.u * q.u + q.v * q.v));
+    return q;
+}
+
+// sqrt(gamma max(p, eps) / rho) (Euler2D.jl:35-39)
+template <typename R>
+__device__ __forceinline__ R sound_speed(const Prim<R>& q, R gamma) {
+    return rsqrt_(gamma * rmax(q.p, R(kEps64)) / q.rho);
+}
+
+// physical fluxes F (x) and G (y) (Euler2D.jl:54-79)
+template <typename R>
+__device__ __forceinline__ void phys_flux(const R U[4], const Prim<R>& q, R F[4], R G[4]) {
+    F[0] = U[1];
+    F[1] = U[1] * q.u + q.p;
+    F[2] = U[2] * q.u;
+    F[3] = (U[3] + q.p) * q.u;
+    G[0] = U[2];
+    G[1] = U[1] * q.v;
+    G[2] = U[2] * q.v + q.p;
+    G[3] = (U[3] + q.p) * q.v;
+}
+
+// Rusanov normal flux (Euler2D.jl:114-151)
+template <typename R>
+__device__ __forceinline__ void rusanov_n(const R uL[4], const R uR[4], R nx, R ny, R gamma,
+                                          R out[4]) {
+    Prim<R> a = primitives(uL, gamma);
+    Prim<R> b = primitives(uR, gamma);
+    R fL1 = uL[1] * nx + uL[2] * ny;
+    R fL2 = (uL[1] * a.u + a.p) * nx + (uL[1] * a.v) * ny;
+    R fL3 = (uL[2] * a.u) * nx + (uL[2] * a.v + a.p) * ny;
+    R fL4 = (uL[3] + a.p) * (a.u * nx + a.v * ny);
+    R fR1 = uR[1] * nx + uR[2] * ny;
+    R fR2 = (uR[1] * b.u + b.p) * nx + (uR[1] * b.v) * ny;
+    R fR3 = (uR[2] * b.u) * nx + (uR[2] * b.v + b.p) * ny;
+    R fR4 = (uR[3] + b.p) * (b.u * nx + b.v * ny);
+    R sL = rabs(a.u * nx + a.v * ny) + sound_speed(a, gamma);
+    R sR = rabs(b.u * nx + b.v * ny) + sound_speed(b, gamma);
+    R hl = R(0.5) * rmax(sL, sR);
+    out[0] = R(0.5) * (fL1 + fR1) - hl * (uR[0] - uL[0]);
+    out[1] = R(0.5) * (fL2 + fR2) - hl * (uR[1] - uL[1]);
+    out[2] = R(0.5) * (fL3 + fR3) - hl * (uR[2] - uL[2]);
+    out[3] = R(0.5) * (fL4 + fR4) - hl * (uR[3] - uL[3]);
+}
+
+// HLLC normal flux with Davis wave speeds (HLLC.jl:235-305)
+template <typename R>
+__device__ __forceinline__ void hllc_n(const R uL[4], const R uR[4], R nx0, R ny0, R gamma,
+                                       R out[4]) {
+    R nrm = rhypot(nx0, ny0);
+    if (nrm < R(kEps64)) {
+        rusanov_n(uL, uR, nx0, ny0, gamma, out);
+        return;
+    }
+    R nx = nx0 / nrm, ny = ny0 / nrm;
+    Prim<R> a = primitives(uL, gamma);
+    Prim<R> b = primitives(uR, gamma);
+    R pL = rmax(a.p, R(kEps64)), pR = rmax(b.p, R(kEps64));
+    R unL = a.u * nx + a.v * ny, unR = b.u * nx + b.v * ny;
+    R cL = rsqrt_(gamma * pL / a.rho), cR = rsqrt_(gamma * pR / b.rho);
+    R SL = rmin(unL - cL, unR - cR);
+    R SR = rmax(unL + cL, unR + cR);
+    R num = pR - pL + a.rho * unL * (SL - unL) - b.rho * unR * (SR - unR);
+    R den = a.rho * (SL - unL) - b.rho * (SR - unR);
+    R SM = rabs(den) < R(kEps64) ? R(0.5) * (unL + unR) : num / den;
+    // F.n with the unfloored pressure (physical_flux_x/y use pressure without the floor)
+    R fL[4], fR[4];
+    {
+        R F[4], G[4];
+        phys_flux(uL, a, F, G);
+        for (int c = 0; c < 4; ++c) fL[c] = F[c] * nx + G[c] * ny;
+        phys_flux(uR, b, F, G);
+        for (int c = 0; c < 4; ++c) fR[c] = F[c] * nx + G[c] * ny;
+    }
+    if (R(0) <= SL) {
+        for (int c = 0; c < 4; ++c) out[c] = fL[c];
+        return;
+    }
+    if (R(0) >= SR) {
+        for (int c = 0; c < 4; ++c) out[c] = fR[c];
+        return;
+    }
+    const bool left = (SL <= R(0)) && (R(0) <= SM);
+    const R* U = left ? uL : uR;
+    const R* Fn = left ? fL : fR;
+    const R rho = left ? a.rho : b.rho;
+    const R uu = left ? a.u : b.u;
+    const R vv = left ? a.v : b.v;
+    const R p = left ? pL : pR;
+    const R un = left ? unL : unR;
+    const R S = left ? SL : SR;
+    R us = (uu - un * nx) + SM * nx;
+    R vs = (vv - un * ny) + SM * ny;
+    R factor = rho * (S - un) / (S - SM);
+    R es = U[3] / rho + (SM - un) * (SM + p / (rho * (S - un)));
+    out[0] = Fn[0] + S * (factor - U[0]);
+    out[1] = Fn[1] + S * (factor * us - U[1]);
+    out[2] = Fn[2] + S * (factor * vs - U[2]);
+    out[3] = Fn[3] + S * (factor * es - U[3]);
+}
+
+template <int FLUX, typename R>
+__device__ __forceinline__ void iflux(const R uL[4], const R uR[4], R nx, R ny, R gamma, R out[4]) {
+    if constexpr (FLUX == 1)
+        hllc_n(uL, uR, nx, ny, gamma, out);
+    else
+        rusanov_n(uL, uR, nx, ny, gamma, out);
+}
+
+// Element wave speed term max(|u| + c, |v| + c) at one point (PerssonAV2D.jl:62-82,
+// Euler2D.jl:92-104)
+template <typename R>
+__device__ __forceinline__ R point_wavespeed(const R U[4], R gamma) {
+    Prim<R> q = primitives(U, gamma);
+    R c = sound_speed(q, gamma);
+    return rmax(rabs(q.u) + c, rabs(q.v) + c);
+}
+
+}  // namespace frf

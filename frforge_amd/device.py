@@ -1,0 +1,206 @@
+"""ctypes binding of libfrforge_hip.so (the C ABI declared in include/frforge_hip.h).
+
+The product path: every residual, sensor and time step of the host API goes through this
+library on the GPU.  There is no CPU fallback; a missing or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfrforge_hip.so")
+
+FRF_OK = 0
+ERRORS = {-1: "FRF_E_ARG", -2: "FRF_E_STATE", -3: "FRF_E_HIP", -4: "FRF_E_UNSUPPORTED"}
+BC = {"periodic": 0, "transmissive": 1}
+FLUX = {"rusanov": 0, "hllc": 1}
+PRECISION = {"fp64": 0, "fp32_mixed": 1}
+CAPTURING = {"null": 0, "persson_br0": 1, "persson_dd": 2}
+STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
+
+# Every extern "C" symbol the header declares (checked by the CPU test suite).
+EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
+            "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
+            "frf_run", "frf_advance", "frf_profile_steps", "frf_stream", "frf_last_error",
+            "frf_destroy")
+
+
+class FrfProblem(C.Structure):
+    _fields_ = [("p", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("neq", C.c_int32),
+                ("bc_left", C.c_int32), ("bc_right", C.c_int32), ("bc_bottom", C.c_int32),
+                ("bc_top", C.c_int32), ("flux", C.c_int32), ("precision", C.c_int32),
+                ("capturing", C.c_int32), ("reserved", C.c_int32), ("gamma", C.c_double),
+                ("kappa", C.c_double), ("s0_factor", C.c_double), ("eps_floor", C.c_double),
+                ("c_av", C.c_double)]
+
+
+class FrfRunResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("fail_stage", C.c_int32), ("n_steps", C.c_int64),
+                ("t", C.c_double)]
+
+
+class FrfProfile(C.Structure):
+    _fields_ = [("stage_ms", C.c_double * 3), ("stage_launches", C.c_int64 * 3),
+                ("elem_ms", C.c_double), ("elem_launches", C.c_int64), ("control_ms", C.c_double),
+                ("control_launches", C.c_int64), ("span_ms", C.c_double), ("status", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class FrfError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raise loudly if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FrfError(f"{path} is missing: build it with `python -m frforge_amd.build` "
+                       "(the device path has no CPU fallback)")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.frf_create.argtypes = [C.POINTER(FrfProblem), C.POINTER(vp)]
+    L.frf_set_operators.argtypes = [vp] + [dp] * 8
+    L.frf_set_geometry.argtypes = [vp, dp, dp]
+    L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
+    L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
+    L.frf_residual.argtypes = [vp, dp, C.c_int64]
+    L.frf_sense.argtypes = [vp, dp, dp]
+    L.frf_compute_dt.argtypes = [vp, C.c_double, dp]
+    L.frf_rk3_step.argtypes = [vp, C.c_double, C.POINTER(C.c_int32)]
+    L.frf_run.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int64, C.POINTER(FrfRunResult)]
+    L.frf_advance.argtypes = [vp, C.c_double, C.c_int64, C.POINTER(C.c_int32)]
+    L.frf_profile_steps.argtypes = [vp, C.c_double, C.c_int64, C.POINTER(FrfProfile)]
+    L.frf_stream.restype = vp
+    L.frf_stream.argtypes = [vp]
+    L.frf_last_error.restype = C.c_char_p
+    L.frf_last_error.argtypes = [vp]
+    L.frf_destroy.argtypes = [vp]
+    for name in ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
+                 "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt",
+                 "frf_rk3_step", "frf_run", "frf_advance", "frf_profile_steps", "frf_destroy"):
+        getattr(L, name).restype = C.c_int32
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DeviceContext:
+    """One frf_ctx: a problem (mesh, degree, flux, precision, capturing) resident on the GPU."""
+
+    def __init__(self, *, p, nx, ny, bc_left, bc_right, bc_bottom, bc_top, flux="rusanov",
+                 precision="fp64", capturing="null", gamma=1.4, kappa=4.0, s0_factor=-4.0,
+                 eps_floor=1e-16, c_av=0.1):
+        self.lib = load_library()
+        pb = FrfProblem(p, nx, ny, 4, BC[bc_left], BC[bc_right], BC[bc_bottom], BC[bc_top],
+                        FLUX[flux], PRECISION[precision], CAPTURING[capturing], 0, gamma, kappa,
+                        s0_factor, eps_floor, c_av)
+        h = C.c_void_p()
+        rc = self.lib.frf_create(C.byref(pb), C.byref(h))
+        if rc != FRF_OK:
+            raise FrfError(f"frf_create: {ERRORS.get(rc, rc)}: {self.lib.frf_last_error(None).decode()}")
+        self.h = h
+        self.p, self.nx, self.ny = p, nx, ny
+        self.Np, self.Nel = p + 1, nx * ny
+        self.nstate = self.Np * self.Np * self.Nel * 4
+        self.host_dirty = True
+
+    def _chk(self, rc, what):
+        if rc != FRF_OK:
+            raise FrfError(f"{what}: {ERRORS.get(rc, rc)}: {self.lib.frf_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mark_host_dirty(self):
+        self.host_dirty = True
+
+    def set_operators(self, xi, w, D, ell_L, ell_R, gL_xi, gR_xi, V):
+        arrs = [np.asfortranarray(np.asarray(a, dtype=np.float64)) for a in (xi, w, D, ell_L, ell_R, gL_xi, gR_xi, V)]
+        self._ops_keep = arrs
+        self._chk(self.lib.frf_set_operators(self.h, *[_dp(a) for a in arrs]), "frf_set_operators")
+
+    def set_geometry(self, x_vertices, y_vertices):
+        xv = np.ascontiguousarray(x_vertices, dtype=np.float64)
+        yv = np.ascontiguousarray(y_vertices, dtype=np.float64)
+        self._chk(self.lib.frf_set_geometry(self.h, _dp(xv), _dp(yv)), "frf_set_geometry")
+
+    def _check_state(self, u):
+        if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.f_contiguous
+                and u.size == self.nstate):
+            raise ValueError("state must be a Fortran-ordered float64 array of "
+                             f"{self.nstate} values (Np, Np, Nel, 4)")
+
+    def upload(self, u: np.ndarray, t: float):
+        self._check_state(u)
+        self._chk(self.lib.frf_upload_state(self.h, _dp(u), self.nstate, t), "frf_upload_state")
+        self.host_dirty = False
+
+    def download(self, u: np.ndarray) -> float:
+        self._check_state(u)
+        t = C.c_double()
+        self._chk(self.lib.frf_download_state(self.h, _dp(u), self.nstate, C.byref(t)), "frf_download_state")
+        return t.value
+
+    def residual(self, du: np.ndarray):
+        self._check_state(du)
+        self._chk(self.lib.frf_residual(self.h, _dp(du), self.nstate), "frf_residual")
+        return du
+
+    def sense(self):
+        sigma = np.zeros(self.Nel)
+        eps = np.zeros(self.Nel)
+        self._chk(self.lib.frf_sense(self.h, _dp(sigma), _dp(eps)), "frf_sense")
+        return sigma, eps
+
+    def compute_dt(self, cfl: float) -> float:
+        dt = C.c_double()
+        self._chk(self.lib.frf_compute_dt(self.h, cfl, C.byref(dt)), "frf_compute_dt")
+        return dt.value
+
+    def rk3_step(self, dt: float) -> str:
+        st = C.c_int32()
+        self._chk(self.lib.frf_rk3_step(self.h, dt, C.byref(st)), "frf_rk3_step")
+        return STATUS[st.value]
+
+    def advance(self, dt: float, n_steps: int) -> str:
+        st = C.c_int32()
+        self._chk(self.lib.frf_advance(self.h, dt, n_steps, C.byref(st)), "frf_advance")
+        return STATUS[st.value]
+
+    def run(self, t_final: float, cfl: float, dt=None, max_steps: int = 10**7):
+        res = FrfRunResult()
+        self._chk(self.lib.frf_run(self.h, t_final, cfl, -1.0 if dt is None else float(dt),
+                                   max_steps, C.byref(res)), "frf_run")
+        return {"status": STATUS[res.status], "n_steps": res.n_steps, "t": res.t,
+                "fail_stage": res.fail_stage}
+
+    def profile_steps(self, dt: float, n_steps: int) -> dict:
+        pr = FrfProfile()
+        self._chk(self.lib.frf_profile_steps(self.h, dt, n_steps, C.byref(pr)), "frf_profile_steps")
+        return {"stage_ms": list(pr.stage_ms), "stage_launches": list(pr.stage_launches),
+                "elem_ms": pr.elem_ms, "elem_launches": pr.elem_launches,
+                "control_ms": pr.control_ms, "control_launches": pr.control_launches,
+                "span_ms": pr.span_ms, "status": STATUS[pr.status]}
+
+    def stream(self) -> int:
+        return int(self.lib.frf_stream(self.h) or 0)

@@ -1,0 +1,137 @@
+/*
+ * frforge_hip.h -- C ABI of the MI355X (gfx950) FRForge 2D FR residual and SSP-RK3 time step.
+ *
+ * Drop-in boundary for FRForge's Julia API (RcktMan77/FRForge): each entry point below replaces
+ * one reference function for SolutionState2D / Euler2D on affine meshes with periodic or
+ * transmissive sides.  A Julia `ccall` shim keeps the reference signatures; the mapping is in
+ * INTEGRATION.md.
+ *
+ * Conventions
+ *   - Every function returns int32: FRF_OK (0) or a negative FRF_E_* code; frf_last_error()
+ *     gives the message.  Numeric failure is not an error: it is reported as a status
+ *     (FRF_STATUS_DIVERGED), as the reference reports :diverged (SSP_RK3.jl:148-161).
+ *   - State arrays are float64 in the reference's column-major layout u[i,j,e,c]
+ *     (SolutionState2D.jl:9-18): index i + Np*(j + Np*(e + Nel*c)), Np = p+1, Nel = nx*ny,
+ *     element e = jy*nx + jx (0-based).
+ *   - The context owns all device memory; no host pointer is retained after a call returns.
+ *   - Calls are blocking (they synchronise the context's stream before returning).  A context
+ *     must not be used from two host threads at once.
+ */
+#ifndef FRFORGE_HIP_H
+#define FRFORGE_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRF_OK 0
+#define FRF_E_ARG (-1)          /* invalid argument (reference: ArgumentError / DimensionMismatch) */
+#define FRF_E_STATE (-2)        /* call out of order (operators, geometry or state not set) */
+#define FRF_E_HIP (-3)          /* HIP runtime failure */
+#define FRF_E_UNSUPPORTED (-4)  /* configuration the device path does not implement */
+
+enum { FRF_BC_PERIODIC = 0, FRF_BC_TRANSMISSIVE = 1 };
+enum { FRF_FLUX_RUSANOV = 0, FRF_FLUX_HLLC = 1 };
+enum { FRF_FP64 = 0, FRF_FP32_MIXED = 1 };
+enum { FRF_CAPT_NULL = 0, FRF_CAPT_PERSSON_BR0 = 1, FRF_CAPT_PERSSON_DD = 2 };
+enum { FRF_STATUS_OK = 0, FRF_STATUS_DIVERGED = 1, FRF_STATUS_MAX_STEPS = 2 };
+
+typedef struct frf_ctx frf_ctx;
+
+/* Problem description: Mesh2D sizes and BCs (Mesh2D.jl:35-91), degree (Operators.jl:85),
+ * SchemeConfig.flux (SchemeConfig.jl:15-34), Euler2D.gamma (Euler2D.jl:8-12), the capturing
+ * method and its parameters (PerssonAV.jl:9-69: kappa, s0_factor, eps_floor, c_av, av_form). */
+typedef struct frf_problem {
+    int32_t p;           /* polynomial degree, 1..5 */
+    int32_t nx, ny;      /* elements per direction */
+    int32_t neq;         /* must be 4 (Euler2D) */
+    int32_t bc_left, bc_right, bc_bottom, bc_top;   /* FRF_BC_* */
+    int32_t flux;        /* FRF_FLUX_* */
+    int32_t precision;   /* FRF_FP64 (plain) or FRF_FP32_MIXED */
+    int32_t capturing;   /* FRF_CAPT_* */
+    int32_t reserved;
+    double gamma;
+    double kappa, s0_factor, eps_floor, c_av;
+} frf_problem;
+
+typedef struct frf_run_result {
+    int32_t status;      /* FRF_STATUS_* (reference: :ok / :diverged / :max_steps) */
+    int32_t fail_stage;  /* 1..3 non-finite du in that stage, 4 non-finite u at step end, else 0 */
+    int64_t n_steps;
+    double t;
+} frf_run_result;
+
+/* Create a context (the reference's allocate_state + ensure_residual_workspace!,
+ * SolutionState2D.jl:20-43, ResidualWorkspace.jl:72-80). */
+int32_t frf_create(const frf_problem* problem, frf_ctx** out);
+
+/* Reference-element operators exactly as FRForge's build_operators returns them
+ * (Operators.jl:85-100): xi, w (Np); D (Np*Np, column-major D[i,j] at i + Np*j); l_L, l_R,
+ * gL_xi, gR_xi (Np); V_legendre (Np*Np column-major).  Replaces the ops field of the state. */
+int32_t frf_set_operators(frf_ctx* ctx, const double* xi, const double* w, const double* D,
+                          const double* ell_L, const double* ell_R, const double* gL_xi,
+                          const double* gR_xi, const double* V_legendre);
+
+/* Mesh vertices (Mesh2D.x_vertices, nx+1 values; y_vertices, ny+1 values).  The library
+ * rebuilds the affine metric terms of build_mesh_metrics (MeshMetrics2D.jl:82-218) from them. */
+int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y_vertices);
+
+/* Copy state.u (len = Np*Np*Nel*4 values) and state.t to / from the device. */
+int32_t frf_upload_state(frf_ctx* ctx, const double* u, int64_t len, double t);
+int32_t frf_download_state(frf_ctx* ctx, double* u, int64_t len, double* t);
+
+/* residual!(du, state, eq, method) (Residual2D.jl:131-416) on the uploaded state; du is fully
+ * overwritten, same layout as u. */
+int32_t frf_residual(frf_ctx* ctx, double* du, int64_t len);
+
+/* sense! (PerssonAV2D.jl:20-59) and element_viscosities_2d (:89-109) on the uploaded state.
+ * sigma and eps are Nel values each; either may be NULL. */
+int32_t frf_sense(frf_ctx* ctx, double* sigma, double* eps);
+
+/* compute_dt(state, eq; cfl) (SSP_RK3.jl:126-133). */
+int32_t frf_compute_dt(frf_ctx* ctx, double cfl, double* dt);
+
+/* ssp_rk3_step!(state, eq, method, dt) (SSP_RK3.jl:135-164).  *status receives
+ * FRF_STATUS_OK or FRF_STATUS_DIVERGED; the device state and time are left exactly where the
+ * reference leaves state.u / state.t. */
+int32_t frf_rk3_step(frf_ctx* ctx, double dt, int32_t* status);
+
+/* ssp_rk3!(state, eq, method, t_final; cfl, dt, max_steps) (SSP_RK3.jl:166-211), via
+ * integrate! (Integrate.jl:9-43).  fixed_dt > 0 selects the fixed step, else the CFL step.
+ * The loop runs on the device; the host only polls the step-control block between batches. */
+int32_t frf_run(frf_ctx* ctx, double t_final, double cfl, double fixed_dt, int64_t max_steps,
+                frf_run_result* result);
+
+/* Advance `n_steps` SSP-RK3 steps of fixed size dt with no end time (the timed unit of the
+ * throughput benchmark); status as frf_rk3_step. */
+int32_t frf_advance(frf_ctx* ctx, double dt, int64_t n_steps, int32_t* status);
+
+/* Per-kernel device time of `n_steps` fixed-dt steps (as frf_advance), from HIP event pairs
+ * recorded on the context's stream around every launch.  No counterpart in the reference
+ * (its timings are wall-clock only, CaseReport.jl:49); used by the throughput benchmark. */
+typedef struct frf_profile {
+    double stage_ms[3];          /* summed device time of the stage-1/2/3 kernels */
+    int64_t stage_launches[3];
+    double elem_ms;              /* element sensor / wave-speed passes */
+    int64_t elem_launches;
+    double control_ms;           /* step-control kernels */
+    int64_t control_launches;
+    double span_ms;              /* first launch start to last launch end */
+    int32_t status;
+    int32_t reserved;
+} frf_profile;
+int32_t frf_profile_steps(frf_ctx* ctx, double dt, int64_t n_steps, frf_profile* out);
+
+/* The context's HIP stream (hipStream_t), for event timing by a harness. */
+void* frf_stream(frf_ctx* ctx);
+
+/* Last error message of the context (or of the last failed frf_create when ctx is NULL). */
+const char* frf_last_error(const frf_ctx* ctx);
+
+int32_t frf_destroy(frf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRFORGE_HIP_H */

@@ -1,0 +1,61 @@
+"""Shared set-up for the parity tests: one problem built twice, once for the device library and
+once for the CPU oracle, from the same operator and vertex bits."""
+from __future__ import annotations
+
+import numpy as np
+
+from frforge_amd import (Euler2D, Mesh2D, NullCapturing, PerssonAVMethod, PeriodicBC,
+                         SchemeConfig, TransmissiveBC, allocate_state, build_operators,
+                         set_initial_condition)
+from frforge_amd.cases import riemann2d_ic, seeded_quadrant_states, vortex_ic
+from oracle.oracle import Oracle
+
+EQ = Euler2D(1.4)
+
+
+def oracle_ops(ops):
+    return dict(xi=ops.xi, w=ops.w, D=ops.D, lL=ops.ell_L, lR=ops.ell_R, gLxi=ops.gL_xi,
+                gRxi=ops.gR_xi, V=ops.V_legendre)
+
+
+def method_for(capturing: str, c_av: float = 0.1):
+    if capturing == "null":
+        return NullCapturing()
+    form = "conservative_br0" if capturing == "persson_br0" else "element_local_DD"
+    return PerssonAVMethod.create(c_av=c_av, av_form=form)
+
+
+def make_case(p, nx, ny, *, bc="periodic", flux="rusanov", capturing="null", precision="fp64",
+              ic="vortex", seed=1, domain=None, noise=0.0, c_av=0.1):
+    """Returns (state, method, oracle)."""
+    ops = build_operators(p)
+    B = PeriodicBC if bc == "periodic" else TransmissiveBC
+    if domain is None:
+        domain = (-5.0, 5.0, -5.0, 5.0) if ic == "vortex" else (0.0, 1.0, 0.0, 1.0)
+    mesh = Mesh2D(*domain, nx, ny, left_bc=B(), right_bc=B(), bottom_bc=B(), top_bc=B())
+    state = allocate_state(mesh, ops, scheme=SchemeConfig(flux=flux), precision=precision)
+    if ic == "vortex":
+        set_initial_condition(state, vortex_ic(EQ, x0=0.0, y0=0.0))
+    elif ic == "riemann":
+        cfg = seeded_quadrant_states(seed)
+        set_initial_condition(state, lambda x, y: riemann2d_ic(EQ, x, y, config=cfg))
+    elif ic == "cfg6":
+        set_initial_condition(state, lambda x, y: riemann2d_ic(EQ, x, y, config="cfg6"))
+    else:
+        raise ValueError(ic)
+    if noise > 0:
+        rng = np.random.default_rng(seed)
+        state.u[:, :, :, 0] *= 1.0 + noise * rng.standard_normal(state.u.shape[:3])
+    method = method_for(capturing, c_av)
+    orc = Oracle(p, nx, ny, mesh.x_vertices, mesh.y_vertices, oracle_ops(ops), bc_x=bc, bc_y=bc,
+                 flux=flux, capturing=capturing, c_av=c_av)
+    return state, method, orc
+
+
+def rel_err_per_component(a, b):
+    """max over components of ||a_c - b_c||_inf / ||b_c||_inf (floored at 1e-300)."""
+    out = []
+    for c in range(a.shape[3]):
+        den = max(np.max(np.abs(b[:, :, :, c])), 1e-300)
+        out.append(np.max(np.abs(a[:, :, :, c] - b[:, :, :, c])) / den)
+    return max(out)

@@ -47,7 +47,10 @@ def parse():
     ap.add_argument("--p", type=int, default=P)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null")
-    ap.add_argument("--profile-only", default="", help="time only this form:capturing (for rocprof)")
+    ap.add_argument("--profile-only", default="", help="time only these form:capturing pairs (for rocprof)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (torch) whose HBM bytes are known, to "
+                         "calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -142,6 +145,14 @@ def main():
         print(f"# {prec:10s} {capt:12s} {r['wall_s'] / a.steps * 1e3:8.3f} ms/step  "
               f"stage kernels {sum(r['profile']['stage_ms']) / sum(r['profile']['stage_launches']) * 1e3:8.1f} us/launch",
               file=sys.stderr, flush=True)
+    if a.copy_calibration:
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: 3 x {n * 8} bytes read and written", file=sys.stderr)
     if a.profile_only:
         return 0
 

@@ -29,9 +29,11 @@ struct frf_ctx {
     hipStream_t stream = nullptr;
     double* dU[3] = {nullptr, nullptr, nullptr};
     double* dSigma = nullptr;
-    double* dEps = nullptr;
+    double* dEps[3] = {nullptr, nullptr, nullptr};   // viscosities of dU[0..2]
     double* dCtab = nullptr;
     double* dRtab = nullptr;
+    float* dCtabF = nullptr;           // float copies for the FP32-mixed form
+    float* dRtabF = nullptr;
     DevCtl* dCtl = nullptr;
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
     HostOps ops{};
@@ -116,13 +118,17 @@ extern "C" int32_t frf_create(const frf_problem* pb, frf_ctx** out) {
         if (hipMalloc(&c->dU[k], c->nstate * sizeof(double)) != hipSuccess)
             return cleanup(fail(c, FRF_E_HIP, "hipMalloc of %zu state bytes failed", c->nstate * 8));
     if (hipMalloc(&c->dSigma, c->nel * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->dEps, c->nel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dEps[0], c->nel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dEps[1], c->nel * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dEps[2], c->nel * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dCtab, (size_t)pb->nx * ts * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dRtab, (size_t)pb->ny * ts * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dCtabF, (size_t)pb->nx * ts * sizeof(float)) != hipSuccess ||
+        hipMalloc(&c->dRtabF, (size_t)pb->ny * ts * sizeof(float)) != hipSuccess ||
         hipMalloc(&c->dCtl, sizeof(DevCtl)) != hipSuccess ||
         hipHostMalloc(&c->hCtl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(c, FRF_E_HIP, "device allocation failed"));
-    (void)hipMemsetAsync(c->dEps, 0, c->nel * sizeof(double), c->stream);
+    for (int k = 0; k < 3; ++k) (void)hipMemsetAsync(c->dEps[k], 0, c->nel * sizeof(double), c->stream);
     *out = c;
     return FRF_OK;
 }
@@ -132,9 +138,11 @@ extern "C" int32_t frf_destroy(frf_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < 3; ++k) if (c->dU[k]) (void)hipFree(c->dU[k]);
     if (c->dSigma) (void)hipFree(c->dSigma);
-    if (c->dEps) (void)hipFree(c->dEps);
+    for (int k = 0; k < 3; ++k) if (c->dEps[k]) (void)hipFree(c->dEps[k]);
     if (c->dCtab) (void)hipFree(c->dCtab);
     if (c->dRtab) (void)hipFree(c->dRtab);
+    if (c->dCtabF) (void)hipFree(c->dCtabF);
+    if (c->dRtabF) (void)hipFree(c->dRtabF);
     if (c->dCtl) (void)hipFree(c->dCtl);
     if (c->hCtl) (void)hipHostFree(c->hCtl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -311,6 +319,9 @@ extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* 
     c->hmin = hmin;
     HIPCHK(c, hipMemcpyAsync(c->dCtab, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->dRtab, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<float> ctf(ct.begin(), ct.end()), rtf(rt.begin(), rt.end());
+    HIPCHK(c, hipMemcpyAsync(c->dCtabF, ctf.data(), ctf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dRtabF, rtf.data(), rtf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->geom_set = true;
     return FRF_OK;
@@ -347,31 +358,41 @@ static int32_t ready(frf_ctx* c) {
     return FRF_OK;
 }
 
-static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double* u0, double* uout, int stage) {
+static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double* u0, double* uout,
+                                     int stage, const double* eps_in, double* eps_out, int want_lambda) {
     frf::StageParams P{};
     P.uin = uin;
     P.u0 = u0;
     P.uout = uout;
-    P.eps = c->dEps;
+    P.eps = eps_in;
+    P.eps_out = eps_out;
     P.ctab = c->dCtab;
     P.rtab = c->dRtab;
+    const bool mixed = c->pb.precision == FRF_FP32_MIXED;
+    P.ctabR = mixed ? (const void*)c->dCtabF : (const void*)c->dCtab;
+    P.rtabR = mixed ? (const void*)c->dRtabF : (const void*)c->dRtab;
     P.ctl = c->dCtl;
     P.gamma = c->pb.gamma;
+    P.kappa = c->pb.kappa;
+    P.s0 = c->pb.s0_factor * std::log10((double)(c->pb.p > 1 ? c->pb.p : 1));   // persson_s0
+    P.eps_floor = c->pb.eps_floor;
+    P.c_av = c->pb.c_av;
     P.nx = c->pb.nx;
     P.ny = c->pb.ny;
     P.nel = c->nel;
     P.bcx = c->pb.bc_left == FRF_BC_PERIODIC ? 0 : 1;
     P.bcy = c->pb.bc_bottom == FRF_BC_PERIODIC ? 0 : 1;
     P.stage = stage;
+    P.want_lambda = want_lambda;
     P.p = c->pb.p;
     return P;
 }
 
-static frf::ElemParams elem_params(frf_ctx* c, const double* u, int want_sigma) {
+static frf::ElemParams elem_params(frf_ctx* c, const double* u, int want_sigma, double* eps_out) {
     frf::ElemParams E{};
     E.u = u;
     E.sigma = c->dSigma;
-    E.eps = c->dEps;
+    E.eps = eps_out;
     E.ctab = c->dCtab;
     E.rtab = c->dRtab;
     E.ctl = c->dCtl;
@@ -407,10 +428,10 @@ extern "C" int32_t frf_residual(frf_ctx* c, double* du, int64_t len) {
     if ((r = settle(c))) return r;
     const int prec = c->pb.precision, capt = c->pb.capturing;
     if (capt) {
-        frf::ElemParams E = elem_params(c, c->dU[0], 1);
+        frf::ElemParams E = elem_params(c, c->dU[0], 1, c->dEps[0]);
         HIPCHK(c, frf::launch_elem(c->np, prec, E, c->ops, 0, 0, c->stream));
     }
-    frf::StageParams P = stage_params(c, c->dU[0], c->dU[0], c->dU[1], 0);
+    frf::StageParams P = stage_params(c, c->dU[0], c->dU[0], c->dU[1], 0, c->dEps[0], nullptr, 0);
     HIPCHK(c, frf::launch_stage(c->np, prec, c->pb.flux, capt, P, c->ops, c->stream));
     HIPCHK(c, hipMemcpyAsync(du, c->dU[1], c->nstate * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -421,10 +442,10 @@ extern "C" int32_t frf_sense(frf_ctx* c, double* sigma, double* eps) {
     int32_t r = ready(c);
     if (r) return r;
     if ((r = settle(c))) return r;
-    frf::ElemParams E = elem_params(c, c->dU[0], 1);
+    frf::ElemParams E = elem_params(c, c->dU[0], 1, c->dEps[0]);
     HIPCHK(c, frf::launch_elem(c->np, c->pb.precision, E, c->ops, 0, 0, c->stream));
     if (sigma) HIPCHK(c, hipMemcpyAsync(sigma, c->dSigma, c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (eps) HIPCHK(c, hipMemcpyAsync(eps, c->dEps, c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (eps) HIPCHK(c, hipMemcpyAsync(eps, c->dEps[0], c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FRF_OK;
 }
@@ -447,11 +468,6 @@ static void init_ctl(frf_ctx* c, double t_final, double cfl, double fixed_dt, in
     h.p = c->pb.p;
 }
 
-// One SSP-RK3 step on the device (no host synchronisation): the sequence of launches that a
-// run repeats.  `adaptive` adds the wave-speed pass and the step-control kernel.
-static int32_t enqueue_step(frf_ctx* c, bool with_dt_kernel, bool need_lambda) {
-    const int prec = c->pb.precision, capt = c->pb.capturing, flux = c->pb.flux, np = c->np;
-    double *U0 = c->dU[0], *U1 = c->dU[1], *U2 = c->dU[2];
 #define FRF_TIMED(kind, expr) \
     do {                      \
         prof_mark(c);         \
@@ -459,27 +475,31 @@ static int32_t enqueue_step(frf_ctx* c, bool with_dt_kernel, bool need_lambda) {
         prof_mark(c);         \
         prof_kind(c, kind);   \
     } while (0)
-    if (capt || need_lambda) {
-        frf::ElemParams E = elem_params(c, U0, capt ? 1 : 0);
-        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, need_lambda ? 1 : 0, 1, c->stream));
-    }
+
+// Before the first step of a sequence: element viscosities of u^n (capturing methods) and / or
+// its maximum wave speed (CFL steps).  Later steps get both from the stage-3 epilogue.
+static int32_t enqueue_prepass(frf_ctx* c, bool need_lambda) {
+    const int capt = c->pb.capturing;
+    if (!(capt || need_lambda)) return FRF_OK;
+    frf::ElemParams E = elem_params(c, c->dU[0], capt ? 1 : 0, c->dEps[0]);
+    FRF_TIMED(3, frf::launch_elem(c->np, c->pb.precision, E, c->ops, need_lambda ? 1 : 0, 1, c->stream));
+    return FRF_OK;
+}
+
+// One SSP-RK3 step on the device, no host synchronisation: the launch sequence a run repeats.
+// Viscosity buffers rotate with the stage buffers: E0 <-> u^n, E1 <-> u^(1), E2 <-> u^(2).
+static int32_t enqueue_step(frf_ctx* c, bool with_dt_kernel, bool need_lambda) {
+    const int prec = c->pb.precision, capt = c->pb.capturing, flux = c->pb.flux, np = c->np;
+    double *U0 = c->dU[0], *U1 = c->dU[1], *U2 = c->dU[2];
+    double *E0 = c->dEps[0], *E1 = c->dEps[1], *E2 = c->dEps[2];
     if (with_dt_kernel) FRF_TIMED(4, frf::launch_dt(c->dCtl, c->stream));
-    frf::StageParams P1 = stage_params(c, U0, U0, U1, 1);
+    frf::StageParams P1 = stage_params(c, U0, U0, U1, 1, E0, E1, 0);
     FRF_TIMED(0, frf::launch_stage(np, prec, flux, capt, P1, c->ops, c->stream));
-    if (capt) {
-        frf::ElemParams E = elem_params(c, U1, 1);
-        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, 0, 1, c->stream));
-    }
-    frf::StageParams P2 = stage_params(c, U1, U0, U2, 2);
+    frf::StageParams P2 = stage_params(c, U1, U0, U2, 2, E1, E2, 0);
     FRF_TIMED(1, frf::launch_stage(np, prec, flux, capt, P2, c->ops, c->stream));
-    if (capt) {
-        frf::ElemParams E = elem_params(c, U2, 1);
-        FRF_TIMED(3, frf::launch_elem(np, prec, E, c->ops, 0, 1, c->stream));
-    }
-    frf::StageParams P3 = stage_params(c, U2, U0, U0, 3);
+    frf::StageParams P3 = stage_params(c, U2, U0, U0, 3, E2, E0, need_lambda ? 1 : 0);
     FRF_TIMED(2, frf::launch_stage(np, prec, flux, capt, P3, c->ops, c->stream));
     FRF_TIMED(4, frf::launch_finish(c->dCtl, c->stream));
-#undef FRF_TIMED
     return FRF_OK;
 }
 
@@ -500,6 +520,7 @@ extern "C" int32_t frf_rk3_step(frf_ctx* c, double dt, int32_t* status) {
     c->hCtl->dt = dt;
     c->hCtl->t0 = c->t;
     HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    if ((r = enqueue_prepass(c, false))) return r;
     if ((r = enqueue_step(c, false, false))) return r;
     if ((r = read_ctl(c))) return r;
     c->t = c->hCtl->t;
@@ -516,6 +537,7 @@ extern "C" int32_t frf_advance(frf_ctx* c, double dt, int64_t n_steps, int32_t* 
     if ((r = settle(c))) return r;
     init_ctl(c, INFINITY, 0.0, dt, n_steps > 0 ? n_steps : 1);
     HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    if ((r = enqueue_prepass(c, false))) return r;
     for (int64_t s = 0; s < n_steps; ++s)
         if ((r = enqueue_step(c, true, false))) return r;
     if ((r = read_ctl(c))) return r;
@@ -537,6 +559,7 @@ extern "C" int32_t frf_run(frf_ctx* c, double t_final, double cfl, double fixed_
     init_ctl(c, t_final, cfl, fixed_dt > 0.0 ? fixed_dt : 0.0, max_steps);
     HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
     const bool adaptive = !(fixed_dt > 0.0);
+    if ((r = enqueue_prepass(c, adaptive))) return r;
     int64_t batch = 4;
     while (true) {
         for (int64_t s = 0; s < batch; ++s)
@@ -563,7 +586,7 @@ extern "C" int32_t frf_compute_dt(frf_ctx* c, double cfl, double* dt) {
     if ((r = settle(c))) return r;
     init_ctl(c, INFINITY, cfl, 0.0, 1);
     HIPCHK(c, hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
-    frf::ElemParams E = elem_params(c, c->dU[0], 0);
+    frf::ElemParams E = elem_params(c, c->dU[0], 0, nullptr);
     HIPCHK(c, frf::launch_elem(c->np, c->pb.precision, E, c->ops, 1, 0, c->stream));
     if ((r = read_ctl(c))) return r;
     double lam;
@@ -587,6 +610,7 @@ extern "C" int32_t frf_profile_steps(frf_ctx* c, double dt, int64_t n_steps, frf
     c->prof = true;
     c->prof_ev.clear();
     c->prof_kind.clear();
+    r = enqueue_prepass(c, false);
     for (int64_t s = 0; s < n_steps && r == FRF_OK; ++s) r = enqueue_step(c, true, false);
     c->prof = false;
     int32_t r2 = read_ctl(c);

@@ -1,0 +1,127 @@
+// Device helpers shared by the stage and element kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "frf_kernels.h"
+#include "frf_physics.h"
+
+namespace frf {
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs.  Remap so that each
+// XCD walks a contiguous range of tiles: a tile's y-neighbours are then processed on the same
+// XCD shortly before or after it and their halo lines are served by that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    const int per = nblk >> 3, rem = nblk & 7;
+    const int x = bid & 7, k = bid >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+// Row j (Np contiguous values per component) and column i (stride Np) of element e of a state
+// stored as u[i + Np*(j + Np*(e + Nel*c))].
+template <int NP, typename R>
+__device__ __forceinline__ void load_row(const double* __restrict__ u, int nel, int e, int j,
+                                         R out[NP][kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        const double* p = u + ((size_t)c * nel + e) * (NP * NP) + j * NP;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) out[i][c] = R(p[i]);
+    }
+}
+
+template <int NP, typename R>
+__device__ __forceinline__ void load_col(const double* __restrict__ u, int nel, int e, int i,
+                                         R out[NP][kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        const double* p = u + ((size_t)c * nel + e) * (NP * NP) + i;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) out[j][c] = R(p[j * NP]);
+    }
+}
+
+// Lagrange trace of a line: sum_k w[k] v[k][c], accumulated from zero in index order.
+template <int NP, typename R, typename W>
+__device__ __forceinline__ void trace(const W* w, const R v[NP][kNeq], R out[kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) s += R(w[k]) * v[k][c];
+        out[c] = s;
+    }
+}
+
+// Reference-space derivative along a line: out[m][c] = scale * sum_k D[m,k] v[k][c].
+template <int NP, typename R, typename W>
+__device__ __forceinline__ void line_deriv(const W* D, const R v[NP][kNeq], R scale,
+                                           R out[NP][kNeq]) {
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            R s = R(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) s += R(D[m * NP + k]) * v[k][c];
+            out[m][c] = scale * s;
+        }
+}
+
+// BR0 viscous flux along a line: g = eps * (D u) / J  (PerssonAV2D.jl:241-242)
+template <int NP, typename R, typename W>
+__device__ __forceinline__ void br0_line_flux(const W* D, const R v[NP][kNeq], double eps,
+                                              double J, R out[NP][kNeq]) {
+    const R ej = R(eps / J);
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            R s = R(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) s += R(D[m * NP + k]) * v[k][c];
+            out[m][c] = ej * s;
+        }
+}
+
+// Unit-normal norm of a face from its two tangential metric components (MeshMetrics2D.jl:140-175).
+template <typename R>
+__device__ __forceinline__ R face_s(R a, R b) {   // sqrt(a^2 + b^2), floored at eps
+    R s = rsqrt_(a * a + b * b);
+    return rmax(s, R(kEps64));
+}
+
+// SSP-RK3 combine in FP64 (SSP_RK3.jl:149,154,159), reference operation order, no FMA.
+__device__ __forceinline__ double rk_combine(int stage, double u0, double uk, double du, double dt) {
+#pragma clang fp contract(off)
+    if (stage == 1) return u0 + dt * du;
+    if (stage == 2) return (3.0 / 4.0) * u0 + (1.0 / 4.0) * uk + ((1.0 / 4.0) * dt) * du;
+    return (1.0 / 3.0) * u0 + (2.0 / 3.0) * uk + ((2.0 / 3.0) * dt) * du;
+}
+
+// BR0 interior penalty and averaged flux (SharedAV.jl:29-42)
+template <typename R>
+__device__ __forceinline__ R br0_tau(double em, double ep, double hm, double hp, int p) {
+    double ebar = 0.5 * (em + ep);
+    double hbar = 0.5 * (hm + hp);
+    return R(2.0 * ebar * double(p + 1) / (hbar > kEps64 ? hbar : kEps64));
+}
+template <typename R>
+__device__ __forceinline__ R br0_avg(R gm, R gp, R um, R up, R tau) {
+    return R(0.5) * (gm + gp) - tau * (up - um);
+}
+
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// Persson indicator from modal energies (SharedAV.jl:9-21)
+__device__ __forceinline__ double persson_sigma(double eh, double et, double s0, double kappa,
+                                                double floor_) {
+    double se = log10(eh / et + floor_);
+    return 1.0 / (1.0 + exp(-kappa * (se - s0)));
+}
+__device__ __forceinline__ float persson_sigma(float eh, float et, double s0, double kappa,
+                                               double floor_) {
+    float se = log10f(eh / et + float(floor_));
+    return 1.0f / (1.0f + expf(-float(kappa) * (se - float(s0))));
+}
+
+}  // namespace frf

@@ -30,13 +30,13 @@ struct DevCtl {
     int32_t pad;
 };
 
-// Operators, passed by value (kernel-argument segment, read through scalar loads).
-template <int NP>
+// Operators in the arithmetic type, passed by value (kernel-argument segment, scalar loads).
+template <int NP, typename R = double>
 struct OpsT {
-    double D[NP * NP];      // D[i*NP + k] = D[i, k]
-    double lL[NP], lR[NP];  // Lagrange basis at xi = -1, +1
-    double gL[NP], gR[NP];  // g_DG derivatives at the SPs (gL_xi, gR_xi)
-    double Vinv[NP * NP];   // inverse Legendre Vandermonde, Vinv[a*NP + i]
+    R D[NP * NP];      // D[i*NP + k] = D[i, k]
+    R lL[NP], lR[NP];  // Lagrange basis at xi = -1, +1
+    R gL[NP], gR[NP];  // g_DG derivatives at the SPs (gL_xi, gR_xi)
+    R Vinv[NP * NP];   // inverse Legendre Vandermonde, Vinv[a*NP + i]
 };
 
 // Per-column (jx) and per-row (jy) metric tables of an affine mesh; see frf_capi.hip.
@@ -64,13 +64,18 @@ struct StageParams {
     const double* u0;     // u^n (stages 2 and 3)
     double* uout;         // next stage state, or du for a residual-only launch
     const double* eps;    // element viscosities of uin (capturing methods)
+    double* eps_out;      // element viscosities of uout (capturing methods, stages 1..3)
     const double* ctab;
     const double* rtab;
+    const void* ctabR;    // the same tables in the arithmetic type (float for the mixed form)
+    const void* rtabR;
     DevCtl* ctl;
     double gamma;
+    double kappa, s0, eps_floor, c_av;   // Persson sensor / element viscosity parameters
     int nx, ny, nel;
     int bcx, bcy;         // 0 periodic, 1 transmissive
     int stage;            // 0 residual only; 1..3 SSP-RK3 stage
+    int want_lambda;      // accumulate the maximum wave speed of uout into ctl->lam_bits
     int nblk;
     int p;
 };

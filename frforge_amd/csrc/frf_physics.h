@@ -17,8 +17,20 @@ __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R rabs(R a) { return a < R(0) ? -a : a; }
 
+// Reciprocals.  FP32-mixed form: the hardware reciprocal (1 ulp).
+// FP64: hardware reciprocal refined by two Newton steps (within an ulp of 1/a for the normal
+// positive arguments it is used on: densities, Jacobians, face norms).
+__device__ __forceinline__ double rcp_(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-a, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_(float a) { return __builtin_amdgcn_rcpf(a); }
+
 __device__ __forceinline__ double rsqrt_(double a) { return sqrt(a); }
-__device__ __forceinline__ float rsqrt_(float a) { return sqrtf(a); }
+__device__ __forceinline__ float rsqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double rhypot(double a, double b) { return hypot(a, b); }
 __device__ __forceinline__ float rhypot(float a, float b) { return hypotf(a, b); }
 
@@ -27,6 +39,7 @@ __device__ __forceinline__ float rhypot(float a, float b) { return hypotf(a, b);
 template <typename R>
 struct Prim {
     R rho;   // max(U1, eps)
+    R inv;   // 1 / rho
     R u, v;  // velocities
     R p;     // pressure (no floor)
 };
@@ -35,9 +48,9 @@ template <typename R>
 __device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
     Prim<R> q;
     q.rho = rmax(U[0], R(kEps64));
-    R inv = R(1) / q.rho;
-    q.u = U[1] * inv;
-    q.v = U[2] * inv;
+    q.inv = rcp_(q.rho);
+    q.u = U[1] * q.inv;
+    q.v = U[2] * q.inv;
     q.p = (gamma - R(1)) * (U[3] - R(0.5) * q.rho * (q.u * q.u + q.v * q.v));
     return q;
 }
@@ -45,7 +58,7 @@ __device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
 // sqrt(gamma max(p, eps) / rho) (Euler2D.jl:35-39)
 template <typename R>
 __device__ __forceinline__ R sound_speed(const Prim<R>& q, R gamma) {
-    return rsqrt_(gamma * rmax(q.p, R(kEps64)) / q.rho);
+    return rsqrt_(gamma * rmax(q.p, R(kEps64)) * q.inv);
 }
 
 // physical fluxes F (x) and G (y) (Euler2D.jl:54-79)
@@ -93,12 +106,13 @@ __device__ __forceinline__ void hllc_n(const R uL[4], const R uR[4], R nx0, R ny
         rusanov_n(uL, uR, nx0, ny0, gamma, out);
         return;
     }
-    R nx = nx0 / nrm, ny = ny0 / nrm;
+    const R inrm = rcp_(nrm);
+    R nx = nx0 * inrm, ny = ny0 * inrm;
     Prim<R> a = primitives(uL, gamma);
     Prim<R> b = primitives(uR, gamma);
     R pL = rmax(a.p, R(kEps64)), pR = rmax(b.p, R(kEps64));
     R unL = a.u * nx + a.v * ny, unR = b.u * nx + b.v * ny;
-    R cL = rsqrt_(gamma * pL / a.rho), cR = rsqrt_(gamma * pR / b.rho);
+    R cL = rsqrt_(gamma * pL * a.inv), cR = rsqrt_(gamma * pR * b.inv);
     R SL = rmin(unL - cL, unR - cR);
     R SR = rmax(unL + cL, unR + cR);
     R num = pR - pL + a.rho * unL * (SL - unL) - b.rho * unR * (SR - unR);

@@ -1,0 +1,682 @@
+// Fused FR residual + SSP-RK3 stage kernel for gfx950 (one launch per stage).
+//
+// A workgroup owns a TX x TY tile of elements and Np "line threads" per element.
+//   P0  The tile's state is read once, coalesced, into LDS (consecutive lanes read consecutive
+//       doubles of an element row).  Concurrently a few threads read the face-adjacent halo
+//       elements (one line each) and reduce them to the Lagrange traces the tile-edge faces need,
+//       so a halo element is never stored.
+//   P1  Thread t of an element sweeps column i = t (eta: G~, D G~, S/N traces) and row j = t
+//       (xi: F~, D F~, W/E traces) out of LDS; all line arithmetic stays in registers.
+//   P2  Every face flux is evaluated once, by the element on its west (south) side, with that
+//       element's normal, exactly as the reference assigns face ownership (Residual2D.jl:173-334),
+//       and shared through LDS.
+//   P3  g_DG corrections, the eta part handed from column owners to row owners through LDS, the
+//       FP64 SSP-RK3 combine and the non-finite checks; stage 3 writes u^(n+1) over u^n.
+//   P4  Epilogue on the new state: Persson modal sensor and element viscosity for the next
+//       stage (written to a second viscosity buffer), and the maximum wave speed for the next
+//       CFL step.  This replaces a separate full pass over the state per stage.
+// The arithmetic type R is double (plain form) or float (FP32-mixed form); states, the stage
+// combine and the wave speed for the time step are always FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frf_device.h"
+#include "frf_kernels.h"
+#include "frf_launch.h"
+#include "frf_physics.h"
+
+namespace frf {
+
+// One face point's four components in a component-major LDS array.
+template <typename R>
+struct Lane {
+    const R* p;
+    int stride;
+    __device__ __forceinline__ R operator[](int c) const { return p[c * stride]; }
+};
+
+template <int NP, typename R, int FLUX, int CAPT, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<NP, R> op) {
+    using L = TabLayout<NP>;
+    constexpr int CT = L::STRIDE;
+    constexpr int NT = TX * TY, NPP = NP * NP, LINE = NP * kNeq, NTHR = NT * NP;
+    constexpr bool BR0 = CAPT == 1, DD = CAPT == 2, AV = CAPT != 0;
+    // Element stride in sU, padded so that the Np lanes of the 64/Np elements in a wave fall in
+    // different LDS banks on both the column and the row sweep: stride = Np+1 (mod banks).
+    constexpr int WPV = sizeof(R) / 4, BANKS = 64 / WPV;
+    constexpr int ESTR = kNeq * NPP + ((NP + 1 - (kNeq * NPP) % BANKS) % BANKS + BANKS) % BANKS;
+    static_assert(ESTR % BANKS == (NP + 1) % BANKS, "sU stride");
+    // face arrays are component-major: [c][le*Np + t], consecutive lanes hit consecutive banks
+    constexpr int FS = NT * NP;
+
+    __shared__ R sU[NT * ESTR];                 // tile state; then the eta contribution; then modes
+    __shared__ R sFx[NT * LINE];                // W-face u traces, then E-face fluxes
+    __shared__ R sFy[NT * LINE];                // S-face u traces, then N-face fluxes
+    __shared__ R sHW[TY * LINE], sHE[TY * LINE];   // halo traces: W-halo east, E-halo west
+    __shared__ R sHS[TX * LINE], sHN[TX * LINE];   //              S-halo north, N-halo south
+    __shared__ R sF0x[TY * LINE], sF0y[TX * LINE]; // fluxes on the tile's west / south edge
+    __shared__ R sGx[BR0 ? NT * LINE : 1], sGy[BR0 ? NT * LINE : 1];   // BR0 g traces, then g^
+    __shared__ R sGHW[BR0 ? TY * LINE : 1], sGHE[BR0 ? TY * LINE : 1];
+    __shared__ R sGHS[BR0 ? TX * LINE : 1], sGHN[BR0 ? TX * LINE : 1];
+    __shared__ R sG0x[BR0 ? TY * LINE : 1], sG0y[BR0 ? TX * LINE : 1];
+    __shared__ double sEpsT[AV ? NT : 1];                     // element viscosities of the tile
+    __shared__ double sEpsH[BR0 ? 2 * (TX + TY) : 1];         // ... of the halo elements
+    __shared__ double sRed[NTHR * 3];                         // epilogue reductions
+
+    if (P.stage > 0) {
+        // Skip after a failure in an earlier stage of this step: reads only values written by
+        // earlier launches (this stage reports P.stage or 4, never less), so it is uniform.
+        if (P.ctl->done != 0 || P.ctl->fail_stage < P.stage) return;
+    }
+    const int nx = P.nx, ny = P.ny, nel = P.nel;
+    const int ntx = (nx + TX - 1) / TX;
+    const int blk = xcd_remap(blockIdx.x, P.nblk);
+    const int tx0 = (blk % ntx) * TX, ty0 = (blk / ntx) * TY;
+    const int nvx = (nx - tx0 < TX) ? nx - tx0 : TX;
+    const int nvy = (ny - ty0 < TY) ? ny - ty0 : TY;
+    const int tid = threadIdx.x;
+    const int le = tid / NP, t = tid - le * NP;
+    const int lx = le % TX, ly = le / TX;
+    const int jx = tx0 + lx, jy = ty0 + ly;
+    const bool valid = lx < nvx && ly < nvy;
+    const int e = jy * nx + jx;
+    const R g = R(P.gamma);
+    const double* ct = P.ctab + (size_t)(valid ? jx : tx0) * CT;   // FP64 tables (widths, AV)
+    const double* rt = P.rtab + (size_t)(valid ? jy : ty0) * CT;
+    const R* ctabR = static_cast<const R*>(P.ctabR);                 // tables in the arithmetic type
+    const R* rtabR = static_cast<const R*>(P.rtabR);
+    const R* ctR = ctabR + (size_t)(valid ? jx : tx0) * CT;
+    const R* rtR = rtabR + (size_t)(valid ? jy : ty0) * CT;
+
+    // neighbours of the tile (uniform over the block)
+    const int jxW = tx0 > 0 ? tx0 - 1 : (P.bcx == 0 ? nx - 1 : -1);
+    const int jxE = tx0 + nvx < nx ? tx0 + nvx : (P.bcx == 0 ? 0 : -1);
+    const int jyS = ty0 > 0 ? ty0 - 1 : (P.bcy == 0 ? ny - 1 : -1);
+    const int jyN = ty0 + nvy < ny ? ty0 + nvy : (P.bcy == 0 ? 0 : -1);
+
+    // ------------------------------------------------------------------ P0: loads
+    {
+        const int rowvals = nvx * NPP;
+        for (int cr = 0; cr < kNeq * nvy; ++cr) {
+            const int c = cr / nvy, lyy = cr - c * nvy;
+            const double* src = P.uin + ((size_t)c * nel + (size_t)(ty0 + lyy) * nx + tx0) * NPP;
+            R* dst = sU + (lyy * TX) * ESTR + c * NPP;
+            for (int q = tid; q < rowvals; q += NTHR) {
+                const int lxx = q / NPP, s = q - lxx * NPP;
+                dst[lxx * ESTR + s] = R(src[q]);
+            }
+        }
+        if constexpr (AV) {
+            if (t == 0) sEpsT[le] = valid ? P.eps[e] : 0.0;
+        }
+        // halo tasks: one line of a face-adjacent element outside the tile
+        int k = tid;
+        if (k < TY * NP) {                                   // west halo: east trace of row j
+            const int lyy = k / NP, j = k - lyy * NP;
+            if (lyy < nvy && jxW >= 0) {
+                const int en = (ty0 + lyy) * nx + jxW;
+                R v[NP][kNeq];
+                load_row<NP>(P.uin, nel, en, j, v);
+                R tr[kNeq];
+                trace<NP>(op.lR, v, tr);
+                for (int c = 0; c < kNeq; ++c) sHW[c * (TY * NP) + lyy * NP + j] = tr[c];
+                if constexpr (BR0) {
+                    const double epsn = P.eps[en];
+                    R gl[NP][kNeq];
+                    br0_line_flux<NP>(op.D, v, epsn, P.ctab[(size_t)jxW * CT + L::JAC], gl);
+                    trace<NP>(op.lR, gl, tr);
+                    for (int c = 0; c < kNeq; ++c) sGHW[c * (TY * NP) + lyy * NP + j] = tr[c];
+                    if (j == 0) sEpsH[lyy] = epsn;
+                }
+            }
+        } else if ((k -= TY * NP) < TY * NP) {               // east halo: west trace of row j
+            const int lyy = k / NP, j = k - lyy * NP;
+            if (lyy < nvy && jxE >= 0) {
+                const int en = (ty0 + lyy) * nx + jxE;
+                R v[NP][kNeq];
+                load_row<NP>(P.uin, nel, en, j, v);
+                R tr[kNeq];
+                trace<NP>(op.lL, v, tr);
+                for (int c = 0; c < kNeq; ++c) sHE[c * (TY * NP) + lyy * NP + j] = tr[c];
+                if constexpr (BR0) {
+                    const double epsn = P.eps[en];
+                    R gl[NP][kNeq];
+                    br0_line_flux<NP>(op.D, v, epsn, P.ctab[(size_t)jxE * CT + L::JAC], gl);
+                    trace<NP>(op.lL, gl, tr);
+                    for (int c = 0; c < kNeq; ++c) sGHE[c * (TY * NP) + lyy * NP + j] = tr[c];
+                    if (j == 0) sEpsH[TY + lyy] = epsn;
+                }
+            }
+        } else if ((k -= TY * NP) < TX * NP) {               // south halo: north trace of column i
+            const int lxx = k / NP, i = k - lxx * NP;
+            if (lxx < nvx && jyS >= 0) {
+                const int en = jyS * nx + tx0 + lxx;
+                R v[NP][kNeq];
+                load_col<NP>(P.uin, nel, en, i, v);
+                R tr[kNeq];
+                trace<NP>(op.lR, v, tr);
+                for (int c = 0; c < kNeq; ++c) sHS[c * (TX * NP) + lxx * NP + i] = tr[c];
+                if constexpr (BR0) {
+                    const double epsn = P.eps[en];
+                    R gl[NP][kNeq];
+                    br0_line_flux<NP>(op.D, v, epsn, P.rtab[(size_t)jyS * CT + L::JAC], gl);
+                    trace<NP>(op.lR, gl, tr);
+                    for (int c = 0; c < kNeq; ++c) sGHS[c * (TX * NP) + lxx * NP + i] = tr[c];
+                    if (i == 0) sEpsH[2 * TY + lxx] = epsn;
+                }
+            }
+        } else if ((k -= TX * NP) < TX * NP) {               // north halo: south trace of column i
+            const int lxx = k / NP, i = k - lxx * NP;
+            if (lxx < nvx && jyN >= 0) {
+                const int en = jyN * nx + tx0 + lxx;
+                R v[NP][kNeq];
+                load_col<NP>(P.uin, nel, en, i, v);
+                R tr[kNeq];
+                trace<NP>(op.lL, v, tr);
+                for (int c = 0; c < kNeq; ++c) sHN[c * (TX * NP) + lxx * NP + i] = tr[c];
+                if constexpr (BR0) {
+                    const double epsn = P.eps[en];
+                    R gl[NP][kNeq];
+                    br0_line_flux<NP>(op.D, v, epsn, P.rtab[(size_t)jyN * CT + L::JAC], gl);
+                    trace<NP>(op.lL, gl, tr);
+                    for (int c = 0; c < kNeq; ++c) sGHN[c * (TX * NP) + lxx * NP + i] = tr[c];
+                    if (i == 0) sEpsH[2 * TY + TX + lxx] = epsn;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P1: line sweeps
+    const double eps_e = AV ? sEpsT[le] : 0.0;
+    const double Jx = ct[L::JAC], dx = ct[L::DELTA], Jy = rt[L::JAC], dy = rt[L::DELTA];
+    const R iJx = R(1.0 / Jx), iJy = R(1.0 / Jy);
+    const R* myU = sU + le * ESTR;
+    R volE[NP][kNeq];              // eta: D G~ along column i = t
+    R trS[kNeq], trN[kNeq], gS[kNeq], gN[kNeq];
+    R volX[NP][kNeq];              // xi: D F~ along row j = t
+    R trW[kNeq], trE[kNeq], fL[kNeq], fR[kNeq];
+    R vgy[NP][kNeq], gyS[kNeq], gyN[kNeq];   // BR0 eta pieces
+    R vgx[NP][kNeq], gxW[kNeq], gxE[kNeq];   // BR0 xi pieces
+    R ddx[NP][kNeq];                          // DD xi piece
+    if (valid) {
+        {   // column i = t
+            const int i = t;
+            R uc[NP][kNeq];
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) uc[j][c] = myU[c * NPP + j * NP + i];
+            R Gt[NP][kNeq];
+            const R xxi = ctR[L::XI + i];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                Prim<R> q = primitives(uc[j], g);
+                R F[4], G[4];
+                phys_flux(uc[j], q, F, G);
+                const R yxi = rtR[L::CROSS + j * NP + i];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) Gt[j][c] = -yxi * F[c] + xxi * G[c];
+            }
+            trace<NP>(op.lL, uc, trS);
+            trace<NP>(op.lR, uc, trN);
+            trace<NP>(op.lL, Gt, gS);
+            trace<NP>(op.lR, Gt, gN);
+            line_deriv<NP>(op.D, Gt, R(1), volE);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) sFy[c * FS + (le) * NP + i] = trS[c];
+            if constexpr (BR0) {
+                R gy[NP][kNeq];
+                br0_line_flux<NP>(op.D, uc, eps_e, Jy, gy);
+                trace<NP>(op.lL, gy, gyS);
+                trace<NP>(op.lR, gy, gyN);
+                line_deriv<NP>(op.D, gy, R(1), vgy);
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sGy[c * FS + (le) * NP + i] = gyS[c];
+            }
+            if constexpr (DD) {
+                R v1[NP][kNeq];
+                line_deriv<NP>(op.D, uc, R(1), v1);
+                line_deriv<NP>(op.D, v1, R(eps_e / (Jy * Jy)), vgy);
+            }
+        }
+        {   // row j = t
+            const int j = t;
+            R ur[NP][kNeq];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) ur[i][c] = myU[c * NPP + j * NP + i];
+            R Ft[NP][kNeq];
+            const R yeta = rtR[L::XI + j];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                Prim<R> q = primitives(ur[i], g);
+                R F[4], G[4];
+                phys_flux(ur[i], q, F, G);
+                const R xeta = ctR[L::CROSS + j * NP + i];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) Ft[i][c] = yeta * F[c] - xeta * G[c];
+            }
+            trace<NP>(op.lL, ur, trW);
+            trace<NP>(op.lR, ur, trE);
+            trace<NP>(op.lL, Ft, fL);
+            trace<NP>(op.lR, Ft, fR);
+            line_deriv<NP>(op.D, Ft, R(1), volX);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) sFx[c * FS + (le) * NP + j] = trW[c];
+            if constexpr (BR0) {
+                R gx[NP][kNeq];
+                br0_line_flux<NP>(op.D, ur, eps_e, Jx, gx);
+                trace<NP>(op.lL, gx, gxW);
+                trace<NP>(op.lR, gx, gxE);
+                line_deriv<NP>(op.D, gx, R(1), vgx);
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sGx[c * FS + (le) * NP + j] = gxW[c];
+            }
+            if constexpr (DD) {
+                R v1[NP][kNeq];
+                line_deriv<NP>(op.D, ur, R(1), v1);
+                line_deriv<NP>(op.D, v1, R(eps_e / (Jx * Jx)), ddx);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P2: faces (once each)
+    R fhE[kNeq], fhN[kNeq], ghE[kNeq], ghN[kNeq];
+    if (valid) {
+        {   // east face of row j = t, with this element's east normal
+            const int j = t;
+            const bool in_tile = lx + 1 < nvx;
+            const bool has_n = in_tile || jxE >= 0;
+            const R a = rtR[L::FE + j], b = ctR[L::FE + j];   // y_eta, x_eta at the E face
+            const R s = face_s(a, b);
+            const R is = rcp_(s);
+            const R nxo = a * is, nyo = -b * is;
+            R fh[4];
+            if (has_n) {
+                const Lane<R> up = in_tile ? Lane<R>{&sFx[(le + 1) * NP + j], FS} : Lane<R>{&sHE[ly * NP + j], TY * NP};
+                R upv[kNeq];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) upv[c] = up[c];
+                iflux<FLUX>(trE, upv, nxo, nyo, g, fh);
+                if constexpr (BR0) {
+                    const Lane<R> gp = in_tile ? Lane<R>{&sGx[(le + 1) * NP + j], FS} : Lane<R>{&sGHE[ly * NP + j], TY * NP};
+                    const double epsn = in_tile ? sEpsT[le + 1] : sEpsH[TY + ly];
+                    const int jxn = in_tile ? jx + 1 : jxE;
+                    const R tau = br0_tau<R>(eps_e, epsn, dx, P.ctab[(size_t)jxn * CT + L::DELTA], P.p);
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) ghE[c] = br0_avg(gxE[c], gp[c], trE[c], upv[c], tau);
+                }
+            } else {
+                iflux<FLUX>(trE, trE, nxo, nyo, g, fh);
+                if constexpr (BR0) {
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) ghE[c] = gxE[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) fhE[c] = fh[c] * s;
+            if (lx == 0) {   // west face of the tile's first column
+                R fw[4];
+                R sw;
+                if (jxW >= 0) {
+                    const R aw = rtR[L::FE + j], bw = ctabR[(size_t)jxW * CT + L::FE + j];
+                    sw = face_s(aw, bw);
+                    R um[kNeq];
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) um[c] = sHW[c * (TY * NP) + ly * NP + j];
+                    const R isw = rcp_(sw);
+                    iflux<FLUX>(um, trW, aw * isw, -bw * isw, g, fw);
+                    if constexpr (BR0) {
+                        const R tau = br0_tau<R>(sEpsH[ly], eps_e, P.ctab[(size_t)jxW * CT + L::DELTA], dx, P.p);
+#pragma unroll
+                        for (int c = 0; c < kNeq; ++c)
+                            sG0x[c * (TY * NP) + ly * NP + j] = br0_avg(sGHW[c * (TY * NP) + ly * NP + j], gxW[c], um[c], trW[c], tau);
+                    }
+                } else {
+                    const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];   // own west face
+                    sw = face_s(aw, bw);
+                    const R isw = rcp_(sw);
+                    const R nxw = -aw * isw, nyw = bw * isw;
+                    iflux<FLUX>(trW, trW, -nxw, -nyw, g, fw);
+                    if constexpr (BR0) {
+#pragma unroll
+                        for (int c = 0; c < kNeq; ++c) sG0x[c * (TY * NP) + ly * NP + j] = gxW[c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sF0x[c * (TY * NP) + ly * NP + j] = fw[c] * sw;
+            }
+        }
+        {   // north face of column i = t, with this element's north normal
+            const int i = t;
+            const bool in_tile = ly + 1 < nvy;
+            const bool has_n = in_tile || jyN >= 0;
+            const R a = rtR[L::FN + i], b = ctR[L::FN + i];   // y_xi, x_xi at the N face
+            const R s = face_s(a, b);
+            const R is = rcp_(s);
+            const R nxo = -a * is, nyo = b * is;
+            R fh[4];
+            if (has_n) {
+                const Lane<R> up = in_tile ? Lane<R>{&sFy[(le + TX) * NP + i], FS} : Lane<R>{&sHN[lx * NP + i], TX * NP};
+                R upv[kNeq];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) upv[c] = up[c];
+                iflux<FLUX>(trN, upv, nxo, nyo, g, fh);
+                if constexpr (BR0) {
+                    const Lane<R> gp = in_tile ? Lane<R>{&sGy[(le + TX) * NP + i], FS} : Lane<R>{&sGHN[lx * NP + i], TX * NP};
+                    const double epsn = in_tile ? sEpsT[le + TX] : sEpsH[2 * TY + TX + lx];
+                    const int jyn = in_tile ? jy + 1 : jyN;
+                    const R tau = br0_tau<R>(eps_e, epsn, dy, P.rtab[(size_t)jyn * CT + L::DELTA], P.p);
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) ghN[c] = br0_avg(gyN[c], gp[c], trN[c], upv[c], tau);
+                }
+            } else {
+                iflux<FLUX>(trN, trN, nxo, nyo, g, fh);
+                if constexpr (BR0) {
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) ghN[c] = gyN[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) fhN[c] = fh[c] * s;
+            if (ly == 0) {   // south face of the tile's first row
+                R fs[4];
+                R ss;
+                if (jyS >= 0) {
+                    const R as = rtabR[(size_t)jyS * CT + L::FN + i], bs = ctR[L::FN + i];
+                    ss = face_s(as, bs);
+                    R um[kNeq];
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) um[c] = sHS[c * (TX * NP) + lx * NP + i];
+                    const R iss = rcp_(ss);
+                    iflux<FLUX>(um, trS, -as * iss, bs * iss, g, fs);
+                    if constexpr (BR0) {
+                        const R tau = br0_tau<R>(sEpsH[2 * TY + lx], eps_e, P.rtab[(size_t)jyS * CT + L::DELTA], dy, P.p);
+#pragma unroll
+                        for (int c = 0; c < kNeq; ++c)
+                            sG0y[c * (TX * NP) + lx * NP + i] = br0_avg(sGHS[c * (TX * NP) + lx * NP + i], gyS[c], um[c], trS[c], tau);
+                    }
+                } else {
+                    const R as = rtR[L::FS + i], bs = ctR[L::FS + i];   // own south face
+                    ss = face_s(as, bs);
+                    const R iss = rcp_(ss);
+                    const R nxs = as * iss, nys = -bs * iss;
+                    iflux<FLUX>(trS, trS, -nxs, -nys, g, fs);
+                    if constexpr (BR0) {
+#pragma unroll
+                        for (int c = 0; c < kNeq; ++c) sG0y[c * (TX * NP) + lx * NP + i] = gyS[c];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sF0y[c * (TX * NP) + lx * NP + i] = fs[c] * ss;
+            }
+        }
+    }
+    __syncthreads();   // every trace read is done: the trace slots now take the fluxes
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            sFx[c * FS + (le) * NP + t] = fhE[c];
+            sFy[c * FS + (le) * NP + t] = fhN[c];
+            if constexpr (BR0) {
+                sGx[c * FS + (le) * NP + t] = ghE[c];
+                sGy[c * FS + (le) * NP + t] = ghN[c];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P3: corrections, combine
+    R* myB = sU + le * ESTR;   // tile state no longer needed: eta contribution [c][j][i]
+    if (valid) {   // column i = t: eta part / J (and the y artificial viscosity, folded in)
+        const int i = t;
+        const Lane<R> fs = ly > 0 ? Lane<R>{&sFy[(le - TX) * NP + i], FS} : Lane<R>{&sF0y[lx * NP + i], TX * NP};
+        const Lane<R> fn = Lane<R>{&sFy[le * NP + i], FS};
+        R fS[kNeq], fN[kNeq], hS[kNeq], hN[kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            fS[c] = fs[c] - gS[c];
+            fN[c] = fn[c] - gN[c];
+        }
+        if constexpr (BR0) {
+            const Lane<R> gs = ly > 0 ? Lane<R>{&sGy[(le - TX) * NP + i], FS} : Lane<R>{&sG0y[lx * NP + i], TX * NP};
+            const Lane<R> gn = Lane<R>{&sGy[le * NP + i], FS};
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                hS[c] = gs[c] - gyS[c];
+                hN[c] = gn[c] - gyN[c];
+            }
+        }
+        const R xxi = ctR[L::XI + i];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const R iJ = rcp_(xxi * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                R v = (volE[j][c] + (fS[c] * op.gL[j] + fN[c] * op.gR[j])) * iJ;
+                if constexpr (BR0) {
+                    R w = (vgy[j][c] + (hS[c] * op.gL[j] + hN[c] * op.gR[j])) * iJy;
+                    v = v - w;
+                } else if constexpr (DD) {
+                    if (eps_e > 0.0) v = v - vgy[j][c];
+                }
+                myB[c * NPP + j * NP + i] = v;   // du gets -v
+            }
+        }
+    }
+    __syncthreads();
+
+    bool bad = false, bad_u = false;
+    double lam_row = 0.0;
+    R rho_new[NP];
+    if (valid) {   // row j = t
+        const int j = t;
+        const Lane<R> fw = lx > 0 ? Lane<R>{&sFx[(le - 1) * NP + j], FS} : Lane<R>{&sF0x[ly * NP + j], TY * NP};
+        const Lane<R> fe = Lane<R>{&sFx[le * NP + j], FS};
+        R fW[kNeq], fE[kNeq], hW[kNeq], hE[kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            fW[c] = fw[c] - fL[c];
+            fE[c] = fe[c] - fR[c];
+        }
+        if constexpr (BR0) {
+            const Lane<R> gw = lx > 0 ? Lane<R>{&sGx[(le - 1) * NP + j], FS} : Lane<R>{&sG0x[ly * NP + j], TY * NP};
+            const Lane<R> ge = Lane<R>{&sGx[le * NP + j], FS};
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                hW[c] = gw[c] - gxW[c];
+                hE[c] = ge[c] - gxE[c];
+            }
+        }
+        const R yeta = rtR[L::XI + j];
+        R du[NP][kNeq];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const R iJ = rcp_(ctR[L::XI + i] * yeta - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                const R a = (volX[i][c] + (fW[c] * op.gL[i] + fE[c] * op.gR[i])) * iJ;
+                R d = -a - myB[c * NPP + j * NP + i];
+                if constexpr (BR0) {
+                    d = d + (vgx[i][c] + (hW[c] * op.gL[i] + hE[c] * op.gR[i])) * iJx;
+                } else if constexpr (DD) {
+                    if (eps_e > 0.0) d = d + ddx[i][c];
+                }
+                du[i][c] = d;
+            }
+        }
+        const int stage = P.stage;
+        const double dt = stage > 0 ? P.ctl->dt : 0.0;
+        double unew[NP][kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            const size_t base = ((size_t)c * nel + e) * NPP + j * NP;
+            if (stage == 0) {
+#pragma unroll
+                for (int i = 0; i < NP; ++i) unew[i][c] = double(du[i][c]);
+            } else {
+                const double* pin = P.uin + base;
+                const double* p0 = P.u0 + base;
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const double d = double(du[i][c]);
+                    bad |= !isfinite(d);
+                    const double uk = pin[i];
+                    const double u0 = (stage == 1) ? uk : p0[i];
+                    unew[i][c] = rk_combine(stage, u0, uk, d, dt);
+                    bad_u |= !isfinite(unew[i][c]);
+                }
+            }
+            double* po = P.uout + base;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) po[i] = unew[i][c];
+        }
+        if (stage > 0 && (P.want_lambda || AV)) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                rho_new[i] = R(unew[i][0]);
+                lam_row = nan_max(lam_row, point_wavespeed<double>(unew[i], P.gamma));
+            }
+        }
+    }
+    if (P.stage > 0) {
+        const int any_bad = __syncthreads_or(bad ? 1 : 0);
+        const int any_bad_u = (P.stage == 3) ? __syncthreads_or(bad_u ? 1 : 0) : 0;
+        if (tid == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
+    }
+
+    // ------------------------------------------------------------------ P4: epilogue
+    if (P.stage == 0 || !(P.want_lambda || AV)) return;
+    __syncthreads();
+    R* sT = sU;   // modal transform scratch: [le][a][j]
+    if (valid && AV) {
+        const int j = t;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            R s = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) s += op.Vinv[a * NP + i] * rho_new[i];
+            sT[le * ESTR + a * NP + j] = s;
+        }
+    }
+    sRed[tid] = lam_row;
+    __syncthreads();
+    if (valid && AV) {
+        const int b = t;
+        R et = R(0), eh = R(0), ecol = R(0);
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            R s = R(0);
+#pragma unroll
+            for (int jj = 0; jj < NP; ++jj) s += sT[le * ESTR + a * NP + jj] * op.Vinv[b * NP + jj];
+            et += s * s;
+            if (a == NP - 1) eh += s * s;
+            else ecol += s * s;
+        }
+        if (b == NP - 1) eh += ecol;
+        sRed[NTHR + tid] = double(et);
+        sRed[2 * NTHR + tid] = double(eh);
+    }
+    __syncthreads();
+    if (valid && t == 0) {
+        double lm = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) lm = nan_max(lm, sRed[le * NP + k]);
+        lm = nan_max(kEps64, lm);
+        if constexpr (AV) {
+            R etot = R(0), ehigh = R(0);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                etot += R(sRed[NTHR + le * NP + k]);
+                ehigh += R(sRed[2 * NTHR + le * NP + k]);
+            }
+            etot += R(P.eps_floor);
+            const double sigma = double(persson_sigma(ehigh, etot, P.s0, P.kappa, P.eps_floor));
+            const double h = dx < dy ? dx : dy;
+            const int pp = P.p > 1 ? P.p : 1;
+            P.eps_out[e] = sigma > 0.0 ? P.c_av * sigma * (h / double(pp)) * lm : 0.0;
+        }
+        sRed[le * NP] = lm;
+    }
+    if (P.want_lambda) {
+        __syncthreads();
+        if (tid == 0) {
+            double m = 0.0;
+            for (int k = 0; k < NT; ++k)
+                if ((k % TX) < nvx && (k / TX) < nvy) m = nan_max(m, sRed[k * NP]);
+            atomicMax(&P.ctl->lam_bits, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launch
+
+template <int NP, typename R>
+static OpsT<NP, R> make_ops(const HostOps& h) {
+    OpsT<NP, R> o;
+    for (int a = 0; a < NP * NP; ++a) {
+        o.D[a] = R(h.D[a]);
+        o.Vinv[a] = R(h.Vinv[a]);
+    }
+    for (int a = 0; a < NP; ++a) {
+        o.lL[a] = R(h.lL[a]);
+        o.lR[a] = R(h.lR[a]);
+        o.gL[a] = R(h.gLxi[a]);
+        o.gR[a] = R(h.gRxi[a]);
+    }
+    return o;
+}
+
+// Tile shapes per degree (elements per tile row x rows); block = TX*TY*Np threads.
+template <int NP> struct Tile { static constexpr int TX = 16, TY = 4; };
+template <> struct Tile<2> { static constexpr int TX = 16, TY = 8; };
+template <> struct Tile<6> { static constexpr int TX = 8, TY = 4; };
+
+template <int NP, typename R, int FLUX, int CAPT>
+static hipError_t launch_stage_t(const StageParams& P0, const HostOps& h, hipStream_t s) {
+    constexpr int TX = Tile<NP>::TX, TY = Tile<NP>::TY;
+    StageParams P = P0;
+    const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
+    P.nblk = ntx * nty;
+    hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY>), dim3(P.nblk), dim3(TX * TY * NP), 0, s, P,
+                       make_ops<NP, R>(h));
+    return hipGetLastError();
+}
+
+template <int NP, typename R>
+static hipError_t launch_stage_fc(int flux, int capt, const StageParams& P, const HostOps& h,
+                                  hipStream_t s) {
+    if (flux == 0) {
+        if (capt == 0) return launch_stage_t<NP, R, 0, 0>(P, h, s);
+        if (capt == 1) return launch_stage_t<NP, R, 0, 1>(P, h, s);
+        return launch_stage_t<NP, R, 0, 2>(P, h, s);
+    }
+    if (capt == 0) return launch_stage_t<NP, R, 1, 0>(P, h, s);
+    if (capt == 1) return launch_stage_t<NP, R, 1, 1>(P, h, s);
+    return launch_stage_t<NP, R, 1, 2>(P, h, s);
+}
+
+template <int NP>
+static hipError_t launch_stage_np(int prec, int flux, int capt, const StageParams& P,
+                                  const HostOps& h, hipStream_t s) {
+    if (prec == 0) return launch_stage_fc<NP, double>(flux, capt, P, h, s);
+    return launch_stage_fc<NP, float>(flux, capt, P, h, s);
+}
+
+hipError_t launch_stage(int np, int prec, int flux, int capt, const StageParams& P,
+                        const HostOps& h, hipStream_t s) {
+    switch (np) {
+        case 2: return launch_stage_np<2>(prec, flux, capt, P, h, s);
+        case 3: return launch_stage_np<3>(prec, flux, capt, P, h, s);
+        case 4: return launch_stage_np<4>(prec, flux, capt, P, h, s);
+        case 5: return launch_stage_np<5>(prec, flux, capt, P, h, s);
+        case 6: return launch_stage_np<6>(prec, flux, capt, P, h, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace frf

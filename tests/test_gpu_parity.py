@@ -59,6 +59,33 @@ def test_residual_persson_fp64(p, bc, capturing):
     assert err <= RTOL_FP64
 
 
+@pytest.mark.parametrize("capturing", ["null", "persson_br0", "persson_dd"])
+@pytest.mark.parametrize("bc", ["periodic", "transmissive"])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_residual_multi_tile(p, bc, capturing):
+    """Meshes spanning several tiles in x and y with partial edge tiles."""
+    state, method, orc = make_case(p, 37, 21, bc=bc, capturing=capturing, ic="riemann", seed=p,
+                                   noise=0.01, flux="hllc" if p % 2 else "rusanov")
+    du, ref = _residual_pair(state, method, orc)
+    err = rel_err_per_component(du, ref)
+    print(f"p={p} {bc} {capturing} 37x21 rel err {err:.3e}", flush=True)
+    assert err <= RTOL_FP64
+
+
+@pytest.mark.parametrize("capturing", ["null", "persson_br0"])
+def test_multi_step_run_multi_tile(capturing):
+    """Several steps exercise the in-kernel viscosity / wave-speed epilogue between stages."""
+    state, method, orc = make_case(3, 37, 21, bc="transmissive", capturing=capturing, ic="riemann",
+                                   seed=12)
+    st_o, n_o, t_o, u_o = orc.run(state.u, 0.0, 0.004, cfl=0.1)
+    r = ssp_rk3(state, EQ, method, 0.004, cfl=0.1)
+    assert (r["status"], r["n_steps"]) == (st_o, n_o) and n_o > 3
+    assert abs(r["t"] - t_o) <= 1e-15
+    err = rel_err_per_component(state.u, u_o)
+    print(f"multi-step {capturing}: n={n_o} rel err {err:.3e}", flush=True)
+    assert err <= 1e-11
+
+
 @pytest.mark.parametrize("p", [2, 3, 4])
 def test_sensor_and_viscosity_fp64(p):
     state, method, orc = make_case(p, 12, 12, bc="transmissive", capturing="persson_br0",

@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null")
     ap.add_argument("--profile-only", default="", help="time only these form:capturing pairs (for rocprof)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 1 GiB device copy (torch) whose HBM bytes are known, to "
+                    help="also run a plain 1 GiB device-to-device copy whose HBM bytes are known, to "
                          "calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
@@ -70,7 +70,7 @@ def build_state(nx, p, precision, capturing):
 
 
 def time_form(nx, p, precision, capturing, steps, warmup, dt):
-    import torch
+    from frforge_amd.device import synchronize
     from frforge_amd.solver import _device, _push
     eq, st, method, _ = build_state(nx, p, precision, capturing)
     dev = _device(st, eq, method)
@@ -79,10 +79,10 @@ def time_form(nx, p, precision, capturing, steps, warmup, dt):
         dt = dev.compute_dt(CFL)
     if warmup > 0:
         assert dev.advance(dt, warmup) == "ok"
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     status = dev.advance(dt, steps)
-    torch.cuda.synchronize()
+    synchronize()
     wall = time.perf_counter() - t0
     assert status == "ok", f"{precision}/{capturing} diverged"
     prof = dev.profile_steps(dt, min(steps, 10))
@@ -127,8 +127,9 @@ def main():
         print(f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU domain decomposition is not part of this build)", file=sys.stderr)
         return 2
-    import torch
-    if not torch.cuda.is_available():
+    # The library brings its own HIP runtime (system ROCm); torch is not loaded in this process.
+    from frforge_amd.device import device_count
+    if device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
     nx, p = a.nx, a.p
@@ -146,13 +147,9 @@ def main():
               f"stage kernels {sum(r['profile']['stage_ms']) / sum(r['profile']['stage_launches']) * 1e3:8.1f} us/launch",
               file=sys.stderr, flush=True)
     if a.copy_calibration:
-        n = (1 << 30) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"# copy calibration: 3 x {n * 8} bytes read and written", file=sys.stderr)
+        from frforge_amd.calibrate import device_copy
+        nbytes = device_copy(1 << 30, repeats=3)
+        print(f"# copy calibration: 3 x {nbytes} bytes read and written", file=sys.stderr)
     if a.profile_only:
         return 0
 

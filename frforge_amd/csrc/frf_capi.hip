@@ -644,3 +644,15 @@ extern "C" int32_t frf_profile_steps(frf_ctx* c, double dt, int64_t n_steps, frf
     out->status = fs < kNoFail ? FRF_STATUS_DIVERGED : FRF_STATUS_OK;
     return FRF_OK;
 }
+
+extern "C" int32_t frf_device_count(int32_t* n) {
+    int k = 0;
+    if (!n) return FRF_E_ARG;
+    if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+    *n = k;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_device_synchronize(void) {
+    return hipDeviceSynchronize() == hipSuccess ? FRF_OK : FRF_E_HIP;
+}

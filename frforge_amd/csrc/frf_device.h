@@ -90,12 +90,20 @@ __device__ __forceinline__ R face_s(R a, R b) {   // sqrt(a^2 + b^2), floored at
     return rmax(s, R(kEps64));
 }
 
-// SSP-RK3 combine in FP64 (SSP_RK3.jl:149,154,159), reference operation order, no FMA.
-__device__ __forceinline__ double rk_combine(int stage, double u0, double uk, double du, double dt) {
+// SSP-RK3 combine in FP64 (SSP_RK3.jl:149,154,159) in the reference's operation order, no FMA:
+//   stage 1: u0 + dt du;  stage 2: (3/4 u0 + 1/4 u1) + (1/4 dt) du;  stage 3: (1/3 u0 + 2/3 u2) + (2/3 dt) du.
+// rk_base forms the parenthesised state blend (stage 1: u^(k) = u0 itself), rk_final adds du.
+__device__ __forceinline__ double rk_base(int stage, double u0, double uk) {
 #pragma clang fp contract(off)
-    if (stage == 1) return u0 + dt * du;
-    if (stage == 2) return (3.0 / 4.0) * u0 + (1.0 / 4.0) * uk + ((1.0 / 4.0) * dt) * du;
-    return (1.0 / 3.0) * u0 + (2.0 / 3.0) * uk + ((2.0 / 3.0) * dt) * du;
+    if (stage == 2) return (3.0 / 4.0) * u0 + (1.0 / 4.0) * uk;
+    if (stage == 3) return (1.0 / 3.0) * u0 + (2.0 / 3.0) * uk;
+    return uk;
+}
+__device__ __forceinline__ double rk_final(int stage, double base, double du, double dt) {
+#pragma clang fp contract(off)
+    if (stage == 1) return base + dt * du;
+    if (stage == 2) return base + ((1.0 / 4.0) * dt) * du;
+    return base + ((2.0 / 3.0) * dt) * du;
 }
 
 // BR0 interior penalty and averaged flux (SharedAV.jl:29-42)

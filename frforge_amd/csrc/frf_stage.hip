@@ -36,7 +36,7 @@ struct Lane {
     __device__ __forceinline__ R operator[](int c) const { return p[c * stride]; }
 };
 
-template <int NP, typename R, int FLUX, int CAPT, int TX, int TY>
+template <int NP, typename R, int FLUX, int CAPT, int TX, int TY, bool PF>
 __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<NP, R> op) {
     using L = TabLayout<NP>;
     constexpr int CT = L::STRIDE;
@@ -47,6 +47,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
     constexpr int WPV = sizeof(R) / 4, BANKS = 64 / WPV;
     constexpr int ESTR = kNeq * NPP + ((NP + 1 - (kNeq * NPP) % BANKS) % BANKS + BANKS) % BANKS;
     static_assert(ESTR % BANKS == (NP + 1) % BANKS, "sU stride");
+    static_assert(TX * TY >= 2 * (TX + TY), "one thread per halo line task");
     // face arrays are component-major: [c][le*Np + t], consecutive lanes hit consecutive banks
     constexpr int FS = NT * NP;
 
@@ -96,17 +97,40 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
     const int jyN = ty0 + nvy < ny ? ty0 + nvy : (P.bcy == 0 ? 0 : -1);
 
     // ------------------------------------------------------------------ P0: loads
+    // Thread t of an element loads row j = t of u^(k) (Np contiguous doubles per component;
+    // consecutive lanes read consecutive rows, so a wave reads contiguous memory) and writes it to
+    // LDS in the arithmetic type.  In FP64 it keeps only the stage-combine base (u^(k) itself, or
+    // the u^n / u^(k) blend of stages 2 and 3, formed here when u^n is prefetched).
+    double base[NP][kNeq];
     {
-        const int rowvals = nvx * NPP;
-        for (int cr = 0; cr < kNeq * nvy; ++cr) {
-            const int c = cr / nvy, lyy = cr - c * nvy;
-            const double* src = P.uin + ((size_t)c * nel + (size_t)(ty0 + lyy) * nx + tx0) * NPP;
-            R* dst = sU + (lyy * TX) * ESTR + c * NPP;
-            for (int q = tid; q < rowvals; q += NTHR) {
-                const int lxx = q / NPP, s = q - lxx * NPP;
-                dst[lxx * ESTR + s] = R(src[q]);
+        double uk[NP][kNeq];
+        if (valid) {
+            load_row<NP>(P.uin, nel, e, t, uk);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) uk[i][c] = 0.0;
+        }
+        bool blended = false;
+        if constexpr (PF) {
+            if (valid && P.stage >= 2) {
+                double u0[NP][kNeq];
+                load_row<NP>(P.u0, nel, e, t, u0);
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) base[i][c] = rk_base(P.stage, u0[i][c], uk[i][c]);
+                blended = true;
             }
         }
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                sU[le * ESTR + c * NPP + t * NP + i] = R(uk[i][c]);
+                if (!blended) base[i][c] = uk[i][c];
+            }
         if constexpr (AV) {
             if (t == 0) sEpsT[le] = valid ? P.eps[e] : 0.0;
         }
@@ -192,14 +216,13 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
     const double eps_e = AV ? sEpsT[le] : 0.0;
     const double Jx = ct[L::JAC], dx = ct[L::DELTA], Jy = rt[L::JAC], dy = rt[L::DELTA];
     const R iJx = R(1.0 / Jx), iJy = R(1.0 / Jy);
-    const R* myU = sU + le * ESTR;
+    R* myU = sU + le * ESTR;
     R volE[NP][kNeq];              // eta: D G~ along column i = t
     R trS[kNeq], trN[kNeq], gS[kNeq], gN[kNeq];
-    R volX[NP][kNeq];              // xi: D F~ along row j = t
+    R rowv[NP][kNeq];              // xi volume part of du along row j = t: -D F~ / J (+ AV_x)
     R trW[kNeq], trE[kNeq], fL[kNeq], fR[kNeq];
-    R vgy[NP][kNeq], gyS[kNeq], gyN[kNeq];   // BR0 eta pieces
-    R vgx[NP][kNeq], gxW[kNeq], gxE[kNeq];   // BR0 xi pieces
-    R ddx[NP][kNeq];                          // DD xi piece
+    R vgy[NP][kNeq], gyS[kNeq], gyN[kNeq];   // AV eta pieces
+    R gxW[kNeq], gxE[kNeq];                   // BR0 xi traces
     if (valid) {
         {   // column i = t
             const int i = t;
@@ -241,6 +264,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                 line_deriv<NP>(op.D, v1, R(eps_e / (Jy * Jy)), vgy);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);   // keep the two sweeps apart: bounds register pressure
         {   // row j = t
             const int j = t;
             R ur[NP][kNeq];
@@ -263,7 +287,16 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
             trace<NP>(op.lR, ur, trE);
             trace<NP>(op.lL, Ft, fL);
             trace<NP>(op.lR, Ft, fR);
-            line_deriv<NP>(op.D, Ft, R(1), volX);
+            {
+                R volX[NP][kNeq];
+                line_deriv<NP>(op.D, Ft, R(1), volX);
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const R iJ = rcp_(ctR[L::XI + i] * yeta - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) rowv[i][c] = -volX[i][c] * iJ;
+                }
+            }
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) sFx[c * FS + (le) * NP + j] = trW[c];
             if constexpr (BR0) {
@@ -271,18 +304,48 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                 br0_line_flux<NP>(op.D, ur, eps_e, Jx, gx);
                 trace<NP>(op.lL, gx, gxW);
                 trace<NP>(op.lR, gx, gxE);
+                R vgx[NP][kNeq];
                 line_deriv<NP>(op.D, gx, R(1), vgx);
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) rowv[i][c] += vgx[i][c] * iJx;
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sGx[c * FS + (le) * NP + j] = gxW[c];
             }
             if constexpr (DD) {
-                R v1[NP][kNeq];
-                line_deriv<NP>(op.D, ur, R(1), v1);
-                line_deriv<NP>(op.D, v1, R(eps_e / (Jx * Jx)), ddx);
+                if (eps_e > 0.0) {
+                    R v1[NP][kNeq], ddx[NP][kNeq];
+                    line_deriv<NP>(op.D, ur, R(1), v1);
+                    line_deriv<NP>(op.D, v1, R(eps_e / (Jx * Jx)), ddx);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i)
+#pragma unroll
+                        for (int c = 0; c < kNeq; ++c) rowv[i][c] += ddx[i][c];
+                }
             }
         }
     }
     __syncthreads();
+    // Every sweep has read the tile: its slots now take the eta volume part of each column,
+    // -(D G~)/J (+ the eta artificial viscosity); P3 adds the face corrections in place.
+    if (valid) {
+        const int i = t;
+        const R xxi = ctR[L::XI + i];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const R iJ = rcp_(xxi * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                R w = volE[j][c] * iJ;
+                if constexpr (BR0) w -= vgy[j][c] * iJy;
+                if constexpr (DD) {
+                    if (eps_e > 0.0) w -= vgy[j][c];
+                }
+                myU[c * NPP + j * NP + i] = w;
+            }
+        }
+    }
 
     // ------------------------------------------------------------------ P2: faces (once each)
     R fhE[kNeq], fhN[kNeq], ghE[kNeq], ghN[kNeq];
@@ -457,13 +520,8 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
             const R iJ = rcp_(xxi * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
-                R v = (volE[j][c] + (fS[c] * op.gL[j] + fN[c] * op.gR[j])) * iJ;
-                if constexpr (BR0) {
-                    R w = (vgy[j][c] + (hS[c] * op.gL[j] + hN[c] * op.gR[j])) * iJy;
-                    v = v - w;
-                } else if constexpr (DD) {
-                    if (eps_e > 0.0) v = v - vgy[j][c];
-                }
+                R v = myB[c * NPP + j * NP + i] + (fS[c] * op.gL[j] + fN[c] * op.gR[j]) * iJ;
+                if constexpr (BR0) v -= (hS[c] * op.gL[j] + hN[c] * op.gR[j]) * iJy;
                 myB[c * NPP + j * NP + i] = v;   // du gets -v
             }
         }
@@ -499,39 +557,39 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
             const R iJ = rcp_(ctR[L::XI + i] * yeta - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
-                const R a = (volX[i][c] + (fW[c] * op.gL[i] + fE[c] * op.gR[i])) * iJ;
-                R d = -a - myB[c * NPP + j * NP + i];
-                if constexpr (BR0) {
-                    d = d + (vgx[i][c] + (hW[c] * op.gL[i] + hE[c] * op.gR[i])) * iJx;
-                } else if constexpr (DD) {
-                    if (eps_e > 0.0) d = d + ddx[i][c];
-                }
+                R d = rowv[i][c] - (fW[c] * op.gL[i] + fE[c] * op.gR[i]) * iJ - myB[c * NPP + j * NP + i];
+                if constexpr (BR0) d += (hW[c] * op.gL[i] + hE[c] * op.gR[i]) * iJx;
                 du[i][c] = d;
             }
         }
         const int stage = P.stage;
         const double dt = stage > 0 ? P.ctl->dt : 0.0;
+        if constexpr (!PF) {
+            if (stage >= 2) {
+                double u0[NP][kNeq];
+                load_row<NP>(P.u0, nel, e, j, u0);
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) base[i][c] = rk_base(stage, u0[i][c], base[i][c]);
+            }
+        }
         double unew[NP][kNeq];
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) {
-            const size_t base = ((size_t)c * nel + e) * NPP + j * NP;
-            if (stage == 0) {
+            const size_t off = ((size_t)c * nel + e) * NPP + j * NP;
 #pragma unroll
-                for (int i = 0; i < NP; ++i) unew[i][c] = double(du[i][c]);
-            } else {
-                const double* pin = P.uin + base;
-                const double* p0 = P.u0 + base;
-#pragma unroll
-                for (int i = 0; i < NP; ++i) {
-                    const double d = double(du[i][c]);
+            for (int i = 0; i < NP; ++i) {
+                const double d = double(du[i][c]);
+                if (stage == 0) {
+                    unew[i][c] = d;
+                } else {
                     bad |= !isfinite(d);
-                    const double uk = pin[i];
-                    const double u0 = (stage == 1) ? uk : p0[i];
-                    unew[i][c] = rk_combine(stage, u0, uk, d, dt);
+                    unew[i][c] = rk_final(stage, base[i][c], d, dt);
                     bad_u |= !isfinite(unew[i][c]);
                 }
             }
-            double* po = P.uout + base;
+            double* po = P.uout + off;
 #pragma unroll
             for (int i = 0; i < NP; ++i) po[i] = unew[i][c];
         }
@@ -632,7 +690,17 @@ static OpsT<NP, R> make_ops(const HostOps& h) {
 }
 
 // Tile shapes per degree (elements per tile row x rows); block = TX*TY*Np threads.
+#ifndef FRF_PREFETCH_U0
+#define FRF_PREFETCH_U0 (-1)   // -1: by capturing kind; 0 / 1 force it off / on
+#endif
+#ifndef FRF_T4X
+#define FRF_T4X 16
+#endif
+#ifndef FRF_T4Y
+#define FRF_T4Y 4
+#endif
 template <int NP> struct Tile { static constexpr int TX = 16, TY = 4; };
+template <> struct Tile<4> { static constexpr int TX = FRF_T4X, TY = FRF_T4Y; };
 template <> struct Tile<2> { static constexpr int TX = 16, TY = 8; };
 template <> struct Tile<6> { static constexpr int TX = 8, TY = 4; };
 
@@ -642,7 +710,9 @@ static hipError_t launch_stage_t(const StageParams& P0, const HostOps& h, hipStr
     StageParams P = P0;
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
     P.nblk = ntx * nty;
-    hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY>), dim3(P.nblk), dim3(TX * TY * NP), 0, s, P,
+    // u^n is prefetched with u^(k) where the stage has enough arithmetic to hide it (capturing)
+    constexpr bool PF = FRF_PREFETCH_U0 < 0 ? (CAPT != 0) : (FRF_PREFETCH_U0 != 0);
+    hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY, PF>), dim3(P.nblk), dim3(TX * TY * NP), 0, s, P,
                        make_ops<NP, R>(h));
     return hipGetLastError();
 }

@@ -24,8 +24,8 @@ STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
 EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
-            "frf_run", "frf_advance", "frf_profile_steps", "frf_stream", "frf_last_error",
-            "frf_destroy")
+            "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
+            "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy")
 
 
 class FrfProblem(C.Structure):
@@ -79,6 +79,10 @@ def load_library(path: str = LIB_PATH):
     L.frf_run.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int64, C.POINTER(FrfRunResult)]
     L.frf_advance.argtypes = [vp, C.c_double, C.c_int64, C.POINTER(C.c_int32)]
     L.frf_profile_steps.argtypes = [vp, C.c_double, C.c_int64, C.POINTER(FrfProfile)]
+    L.frf_device_count.argtypes = [C.POINTER(C.c_int32)]
+    L.frf_device_count.restype = C.c_int32
+    L.frf_device_synchronize.argtypes = []
+    L.frf_device_synchronize.restype = C.c_int32
     L.frf_stream.restype = vp
     L.frf_stream.argtypes = [vp]
     L.frf_last_error.restype = C.c_char_p
@@ -90,6 +94,18 @@ def load_library(path: str = LIB_PATH):
         getattr(L, name).restype = C.c_int32
     _lib = L
     return L
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    load_library().frf_device_count(C.byref(n))
+    return n.value
+
+
+def synchronize() -> None:
+    rc = load_library().frf_device_synchronize()
+    if rc != FRF_OK:
+        raise FrfError("hipDeviceSynchronize failed")
 
 
 def _dp(a: np.ndarray):

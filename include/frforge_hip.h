@@ -123,6 +123,11 @@ typedef struct frf_profile {
 } frf_profile;
 int32_t frf_profile_steps(frf_ctx* ctx, double dt, int64_t n_steps, frf_profile* out);
 
+/* Number of visible GPUs, and a device-wide synchronisation, for harnesses (no reference
+ * counterpart). */
+int32_t frf_device_count(int32_t* n);
+int32_t frf_device_synchronize(void);
+
 /* The context's HIP stream (hipStream_t), for event timing by a harness. */
 void* frf_stream(frf_ctx* ctx);
 

@@ -7,7 +7,7 @@
  * Each function follows the reference Julia source line by line (same loop order, same
  * operation order, no fused multiply-add).  Citations are to RcktMan77/FRForge.
  *
- * How each part is pinned (tests/test_oracle_*.py):
+ * How each part is pinned (tests/test_oracle.py, tests/test_operators.py):
  *   operators           Points.jl/Correction.jl/Operators.jl invariants (test_operators.jl:5-60)
  *                       and numpy's Gauss-Legendre rule; node bits differ from the reference's
  *                       LAPACK Golub-Welsch at the last ulp (reference's LAPACK: parity

@@ -118,7 +118,20 @@ __device__ __forceinline__ R br0_avg(R gm, R gp, R um, R up, R tau) {
     return R(0.5) * (gm + gp) - tau * (up - um);
 }
 
+// The same SSP-RK3 combine for the FP32-mixed form: FP64 throughout, contracted to FMAs (the
+// mixed form is not bit-matched to the reference, so the unfused operation order is not kept).
+__device__ __forceinline__ double rk_base_fma(int stage, double u0, double uk) {
+    if (stage == 2) return __builtin_fma(0.75, u0, 0.25 * uk);
+    if (stage == 3) return __builtin_fma(1.0 / 3.0, u0, (2.0 / 3.0) * uk);
+    return uk;
+}
+// cdt: dt, dt/4 or 2dt/3 for stages 1, 2, 3
+__device__ __forceinline__ double rk_final_fma(double base, double du, double cdt) {
+    return __builtin_fma(cdt, du, base);
+}
+
 __device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+__device__ __forceinline__ float nan_max(float a, float b) { return (b > a || b != b) ? b : a; }
 
 // Persson indicator from modal energies (SharedAV.jl:9-21)
 __device__ __forceinline__ double persson_sigma(double eh, double et, double s0, double kappa,

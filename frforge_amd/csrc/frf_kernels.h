@@ -39,6 +39,17 @@ struct OpsT {
     R Vinv[NP * NP];   // inverse Legendre Vandermonde, Vinv[a*NP + i]
 };
 
+// Operators of the FP32-mixed stage kernel: those of OpsT in float, plus the face traces of the
+// derivative, dL[k] = sum_m lL[m] D[m, k] (so the W/S-face value of D u is dL . u) and dR alike.
+template <int NP>
+struct OpsM {
+    float D[NP * NP];
+    float lL[NP], lR[NP];
+    float gL[NP], gR[NP];
+    float dL[NP], dR[NP];
+    float Vinv[NP * NP];
+};
+
 // Per-column (jx) and per-row (jy) metric tables of an affine mesh; see frf_capi.hip.
 template <int NP>
 struct TabLayout {

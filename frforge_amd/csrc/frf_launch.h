@@ -18,6 +18,9 @@ struct HostOps {
 // capt: 0 = none, 1 = Persson + BR0, 2 = Persson + element-local DD.
 hipError_t launch_stage(int np, int prec, int flux, int capt, const StageParams& P,
                         const HostOps& h, hipStream_t s);
+// the FP32-mixed form (frf_stage_mixed.hip); launch_stage forwards prec != 0 here
+hipError_t launch_stage_mixed(int np, int flux, int capt, const StageParams& P, const HostOps& h,
+                              hipStream_t s);
 hipError_t launch_elem(int np, int prec, const ElemParams& P, const HostOps& h, int accum_lambda,
                        int check_ctl, hipStream_t s);
 hipError_t launch_dt(DevCtl* c, hipStream_t s);

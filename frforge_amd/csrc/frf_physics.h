@@ -162,6 +162,89 @@ __device__ __forceinline__ void iflux(const R uL[4], const R uR[4], R nx, R ny, 
         rusanov_n(uL, uR, nx, ny, gamma, out);
 }
 
+// ---- Axis-aligned normal fluxes (n = e_x for DIR 0, e_y for DIR 1).  On an affine mesh every
+// face normal is a coordinate axis, so these are the fluxes above with n substituted; they serve
+// the FP32-mixed kernel, where the reference's roundoff-sized tangential normal components are
+// below single-precision resolution.
+
+// F . n (Euler2D.jl:54-79) for n = e_DIR; q.p unfloored as in physical_flux_x/y
+template <int DIR, typename R>
+__device__ __forceinline__ void axis_flux(const R U[4], const Prim<R>& q, R out[4]) {
+    constexpr int N = 1 + DIR, T = 2 - DIR;
+    const R un = DIR == 0 ? q.u : q.v;
+    out[0] = U[N];
+    out[N] = U[N] * un + q.p;
+    out[T] = U[T] * un;
+    out[3] = (U[3] + q.p) * un;
+}
+
+// Rusanov (Euler2D.jl:114-151) with n = e_DIR
+template <int DIR, typename R>
+__device__ __forceinline__ void rusanov_axis(const R uL[4], const R uR[4], R gamma, R out[4]) {
+    const Prim<R> a = primitives(uL, gamma), b = primitives(uR, gamma);
+    R fL[4], fR[4];
+    axis_flux<DIR>(uL, a, fL);
+    axis_flux<DIR>(uR, b, fR);
+    const R sL = rabs(DIR == 0 ? a.u : a.v) + sound_speed(a, gamma);
+    const R sR = rabs(DIR == 0 ? b.u : b.v) + sound_speed(b, gamma);
+    const R hl = R(0.5) * rmax(sL, sR);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] = R(0.5) * (fL[c] + fR[c]) - hl * (uR[c] - uL[c]);
+}
+
+// HLLC with Davis wave speeds (HLLC.jl:235-305) with n = e_DIR: the star state's normal
+// momentum carries SM, the tangential one the side's tangential velocity.
+template <int DIR, typename R>
+__device__ __forceinline__ void hllc_axis(const R uL[4], const R uR[4], R gamma, R out[4]) {
+    constexpr int N = 1 + DIR, T = 2 - DIR;
+    const Prim<R> a = primitives(uL, gamma), b = primitives(uR, gamma);
+    const R pL = rmax(a.p, R(kEps64)), pR = rmax(b.p, R(kEps64));
+    const R unL = DIR == 0 ? a.u : a.v, unR = DIR == 0 ? b.u : b.v;
+    const R cL = rsqrt_(gamma * pL * a.inv), cR = rsqrt_(gamma * pR * b.inv);
+    const R SL = rmin(unL - cL, unR - cR);
+    const R SR = rmax(unL + cL, unR + cR);
+    const R num = pR - pL + a.rho * unL * (SL - unL) - b.rho * unR * (SR - unR);
+    const R den = a.rho * (SL - unL) - b.rho * (SR - unR);
+    const R SM = rabs(den) < R(kEps64) ? R(0.5) * (unL + unR) : num * rcp_(den);
+    R fL[4], fR[4];
+    axis_flux<DIR>(uL, a, fL);
+    axis_flux<DIR>(uR, b, fR);
+    if (R(0) <= SL) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = fL[c];
+        return;
+    }
+    if (R(0) >= SR) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = fR[c];
+        return;
+    }
+    const bool left = R(0) <= SM;   // SL < 0 here
+    const R* U = left ? uL : uR;
+    const R* Fn = left ? fL : fR;
+    const R rho = left ? a.rho : b.rho;
+    const R inv = left ? a.inv : b.inv;
+    const R ut = left ? (DIR == 0 ? a.v : a.u) : (DIR == 0 ? b.v : b.u);
+    const R p = left ? pL : pR;
+    const R un = left ? unL : unR;
+    const R S = left ? SL : SR;
+    const R rs = rho * (S - un);
+    const R factor = rs * rcp_(S - SM);
+    const R es = U[3] * inv + (SM - un) * (SM + p * rcp_(rs));
+    out[0] = Fn[0] + S * (factor - U[0]);
+    out[N] = Fn[N] + S * (factor * SM - U[N]);
+    out[T] = Fn[T] + S * (factor * ut - U[T]);
+    out[3] = Fn[3] + S * (factor * es - U[3]);
+}
+
+template <int FLUX, int DIR, typename R>
+__device__ __forceinline__ void iflux_axis(const R uL[4], const R uR[4], R gamma, R out[4]) {
+    if constexpr (FLUX == 1)
+        hllc_axis<DIR>(uL, uR, gamma, out);
+    else
+        rusanov_axis<DIR>(uL, uR, gamma, out);
+}
+
 // Element wave speed term max(|u| + c, |v| + c) at one point (PerssonAV2D.jl:62-82,
 // Euler2D.jl:92-104)
 template <typename R>

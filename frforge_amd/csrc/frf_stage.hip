@@ -730,21 +730,15 @@ static hipError_t launch_stage_fc(int flux, int capt, const StageParams& P, cons
     return launch_stage_t<NP, R, 1, 2>(P, h, s);
 }
 
-template <int NP>
-static hipError_t launch_stage_np(int prec, int flux, int capt, const StageParams& P,
-                                  const HostOps& h, hipStream_t s) {
-    if (prec == 0) return launch_stage_fc<NP, double>(flux, capt, P, h, s);
-    return launch_stage_fc<NP, float>(flux, capt, P, h, s);
-}
-
 hipError_t launch_stage(int np, int prec, int flux, int capt, const StageParams& P,
                         const HostOps& h, hipStream_t s) {
+    if (prec != 0) return launch_stage_mixed(np, flux, capt, P, h, s);
     switch (np) {
-        case 2: return launch_stage_np<2>(prec, flux, capt, P, h, s);
-        case 3: return launch_stage_np<3>(prec, flux, capt, P, h, s);
-        case 4: return launch_stage_np<4>(prec, flux, capt, P, h, s);
-        case 5: return launch_stage_np<5>(prec, flux, capt, P, h, s);
-        case 6: return launch_stage_np<6>(prec, flux, capt, P, h, s);
+        case 2: return launch_stage_fc<2, double>(flux, capt, P, h, s);
+        case 3: return launch_stage_fc<3, double>(flux, capt, P, h, s);
+        case 4: return launch_stage_fc<4, double>(flux, capt, P, h, s);
+        case 5: return launch_stage_fc<5, double>(flux, capt, P, h, s);
+        case 6: return launch_stage_fc<6, double>(flux, capt, P, h, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -112,6 +112,44 @@ def test_residual_mixed_precision(p, capturing):
     assert err <= RTOL_MIXED
 
 
+@pytest.mark.parametrize("capturing", ["null", "persson_br0", "persson_dd"])
+@pytest.mark.parametrize("bc", ["periodic", "transmissive"])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_residual_mixed_multi_tile(p, bc, capturing):
+    """FP32-mixed residual on meshes spanning several tiles with partial edge tiles."""
+    state, method, orc = make_case(p, 37, 21, bc=bc, capturing=capturing, ic="riemann", seed=20 + p,
+                                   noise=0.01, flux="hllc" if p % 2 else "rusanov",
+                                   precision="fp32_mixed")
+    du, ref = _residual_pair(state, method, orc)
+    err = rel_err_per_component(du, ref)
+    print(f"mixed p={p} {bc} {capturing} 37x21 rel err vs fp64 oracle {err:.3e}", flush=True)
+    assert err <= RTOL_MIXED
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_residual_mixed_vortex(p):
+    state, method, orc = make_case(p, 13, 9, ic="vortex", precision="fp32_mixed")
+    du, ref = _residual_pair(state, method, orc)
+    err = rel_err_per_component(du, ref)
+    print(f"mixed p={p} vortex rel err vs fp64 oracle {err:.3e}", flush=True)
+    assert err <= RTOL_MIXED
+
+
+@pytest.mark.parametrize("capturing", ["null", "persson_br0"])
+def test_run_mixed_multi_step(capturing):
+    """Several FP32-mixed steps against the FP64 oracle run: the state stays within FP32
+    resolution of the FP64 solution."""
+    state, method, orc = make_case(3, 37, 21, bc="transmissive", capturing=capturing, ic="riemann",
+                                   seed=12, precision="fp32_mixed")
+    st_o, n_o, t_o, u_o = orc.run(state.u, 0.0, 0.004, cfl=0.1)
+    r = ssp_rk3(state, EQ, method, 0.004, cfl=0.1)
+    assert r["status"] == st_o == "ok"
+    assert abs(r["n_steps"] - n_o) <= 1
+    err = rel_err_per_component(state.u, u_o)
+    print(f"mixed multi-step {capturing}: n={r['n_steps']} (fp64 {n_o}) rel err {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
 @pytest.mark.parametrize("capturing", ["null", "persson_br0"])
 def test_rk3_step_parity(capturing):
     state, method, orc = make_case(3, 10, 8, bc="transmissive", capturing=capturing, ic="riemann",

@@ -16,6 +16,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return x * per + (x < rem ? x : rem) + k;
 }
 
+// Order in which the tiles of an ntx x nty tile grid are visited: vertical bands of W tile
+// columns, each band row by row.  A tile's four neighbours are then visited within about W
+// tiles of it, while their elements are still in the XCD's L2, so the halo lines a tile reads
+// from its neighbours are L2 hits instead of second HBM reads.
+__device__ __forceinline__ void band_tile(int blk, int ntx, int nty, int W, int& tx, int& ty) {
+    const int band = blk / (W * nty);
+    const int r = blk - band * W * nty;
+    const int wb = (ntx - band * W < W) ? ntx - band * W : W;   // the last band may be narrower
+    ty = r / wb;
+    tx = band * W + (r - ty * wb);
+}
+
 // Row j (Np contiguous values per component) and column i (stride Np) of element e of a state
 // stored as u[i + Np*(j + Np*(e + Nel*c))].
 template <int NP, typename R>
@@ -143,6 +155,14 @@ __device__ __forceinline__ float persson_sigma(float eh, float et, double s0, do
                                                double floor_) {
     float se = log10f(eh / et + float(floor_));
     return 1.0f / (1.0f + expf(-float(kappa) * (se - float(s0))));
+}
+// The same in FP32 on the hardware base-2 logarithm and exponential (v_log_f32, v_exp_f32) and
+// reciprocal: log10 x = log2 x log10(2), e^y = 2^(y log2 e).  For the FP32-mixed form.
+__device__ __forceinline__ float persson_sigma_hw(float eh, float et, float s0, float kappa,
+                                                  float floor_) {
+    const float se = __builtin_amdgcn_logf(eh * __builtin_amdgcn_rcpf(et) + floor_) * 0.30102999566398120f;
+    const float ex = __builtin_amdgcn_exp2f(-kappa * (se - s0) * 1.4426950408889634f);
+    return __builtin_amdgcn_rcpf(1.0f + ex);
 }
 
 }  // namespace frf

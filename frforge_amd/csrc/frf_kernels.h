@@ -8,6 +8,10 @@ constexpr int kNeq = 4;
 constexpr int kMaxNp = 6;   // p <= 5
 constexpr int kNoFail = 99; // DevCtl::fail_stage when no stage has failed
 
+#ifndef FRF_BANDW
+#define FRF_BANDW 8         // tile columns per band of the tile visiting order
+#endif
+
 // Step-control block kept in device memory so a whole run can be captured in a HIP graph:
 // the kernels read dt and the status flags from here instead of from kernel arguments.
 struct DevCtl {
@@ -89,6 +93,7 @@ struct StageParams {
     int want_lambda;      // accumulate the maximum wave speed of uout into ctl->lam_bits
     int nblk;
     int p;
+    int resident, ncu;    // blocks of the stage kernel resident per CU, and CUs (start staggering)
 };
 
 struct ElemParams {

@@ -73,7 +73,9 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
     const int nx = P.nx, ny = P.ny, nel = P.nel;
     const int ntx = (nx + TX - 1) / TX;
     const int blk = xcd_remap(blockIdx.x, P.nblk);
-    const int tx0 = (blk % ntx) * TX, ty0 = (blk / ntx) * TY;
+    int btx, bty;
+    band_tile(blk, ntx, (ny + TY - 1) / TY, FRF_BANDW, btx, bty);
+    const int tx0 = btx * TX, ty0 = bty * TY;
     const int nvx = (nx - tx0 < TX) ? nx - tx0 : TX;
     const int nvy = (ny - ty0 < TY) ? ny - ty0 : TY;
     const int tid = threadIdx.x;

@@ -37,6 +37,13 @@
 #include "frf_launch.h"
 #include "frf_physics.h"
 
+#ifndef FRF_STAGGER
+#define FRF_STAGGER 1   // start delay step of the first resident blocks, in s_sleep 127 units
+#endif
+#ifndef FRF_MWAVES
+#define FRF_MWAVES 1   // minimum waves per SIMD asked of the register allocator
+#endif
+
 namespace frf {
 
 // One line (ROW: row `line`, else column `line`) of a halo element reduced to its trace with
@@ -136,7 +143,7 @@ __device__ __forceinline__ void boundary_h(const float ub[kNeq], const float gb[
 }
 
 template <int NP, int FLUX, int CAPT, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, OpsM<NP> op) {
+__global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES) stage_mixed_kernel(StageParams P, OpsM<NP> op) {
     using L = TabLayout<NP>;
     constexpr int CT = L::STRIDE;
     constexpr int NT = TX * TY, NPP = NP * NP, NTHR = NT * NP;
@@ -171,10 +178,22 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, 
         // earlier launches (this stage reports P.stage or 4, never less), so it is uniform.
         if (P.ctl->done != 0 || P.ctl->fail_stage < P.stage) return;
     }
+    if constexpr (FRF_STAGGER > 0) {
+        // The first resident blocks of a CU start together and would run their load and their
+        // compute phases in lockstep.  Blocks are dispatched round-robin over the CUs, so block b
+        // of the first P.ncu * P.resident is the (b / ncu)-th of its CU: delay it by that many
+        // steps, so that the loads of some blocks overlap the compute of others from the start.
+        const int slot = int(blockIdx.x) / P.ncu;
+        if (slot < P.resident)
+            for (int k = 0; k < slot * FRF_STAGGER; ++k) __builtin_amdgcn_s_sleep(127);
+    }
     const int nx = P.nx, ny = P.ny, nel = P.nel;
     const int ntx = (nx + TX - 1) / TX;
     const int blk = xcd_remap(blockIdx.x, P.nblk);
-    const int tx0 = (blk % ntx) * TX, ty0 = (blk / ntx) * TY;
+    const int nty = (ny + TY - 1) / TY;
+    int btx, bty;
+    band_tile(blk, ntx, nty, FRF_BANDW, btx, bty);
+    const int tx0 = btx * TX, ty0 = bty * TY;
     const int nvx = (nx - tx0 < TX) ? nx - tx0 : TX;
     const int nvy = (ny - ty0 < TY) ? ny - ty0 : TY;
     const int tid = threadIdx.x;
@@ -331,6 +350,15 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, 
     __syncthreads();
 
     // ------------------------------------------------------------------ P2: faces (once each)
+    // The row's xi volume term joins the eta one in LDS: sU then holds -(D H)_xi/Jx - (D H)_eta/Jy
+    // per point, and rowV needs no registers through the face phase.
+    if (valid) {
+        float* row = sU + le * ESTR + t * NP;
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) row[c * NPP + i] = -iJy * row[c * NPP + i] - iJx * rowV[i][c];
+    }
     float hE[kNeq], hN[kNeq];
     if (valid) {
         {   // E face of row j = t
@@ -453,8 +481,8 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, 
         float dS[kNeq], dN[kNeq];
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) {
-            dS[c] = hs[c * cs] - HS[c];
-            dN[c] = hN[c] - HN[c];
+            dS[c] = -iJy * (hs[c * cs] - HS[c]);
+            dN[c] = -iJy * (hN[c] - HN[c]);
         }
         float* col = sU + le * ESTR + t;
 #pragma unroll
@@ -487,8 +515,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, 
             double* po = P.uout + ((size_t)c * nel + e) * NPP + t * NP;
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
-                const float d = -iJx * (rowV[i][c] + dW[c] * op.gL[i] + dE[c] * op.gR[i]) -
-                                iJy * row[c * NPP + i];
+                const float d = row[c * NPP + i] - iJx * (dW[c] * op.gL[i] + dE[c] * op.gR[i]);
                 double un;
                 if (stage == 0) {
                     un = double(d);
@@ -568,7 +595,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_mixed_kernel(StageParams P, 
                 ehigh += sT[NPP + 2 * NP + k];
             }
             etot += float(P.eps_floor);
-            const float sigma = persson_sigma(ehigh, etot, P.s0, P.kappa, P.eps_floor);
+            const float sigma = persson_sigma_hw(ehigh, etot, float(P.s0), float(P.kappa), float(P.eps_floor));
             const float h = dx < dy ? dx : dy;
             const int pp = P.p > 1 ? P.p : 1;
             P.eps_out[e] = sigma > 0.0f ? double(float(P.c_av) * sigma * (h / float(pp)) * lm) : 0.0;
@@ -617,10 +644,10 @@ static OpsM<NP> make_ops_mixed(const HostOps& h) {
 
 // Tile shapes per degree (elements per tile row x rows); block = TX*TY*Np threads.
 #ifndef FRF_M4X
-#define FRF_M4X 16
+#define FRF_M4X 8
 #endif
 #ifndef FRF_M4Y
-#define FRF_M4Y 4
+#define FRF_M4Y 8
 #endif
 template <int NP> struct TileM { static constexpr int TX = 16, TY = 4; };
 template <> struct TileM<4> { static constexpr int TX = FRF_M4X, TY = FRF_M4Y; };
@@ -633,6 +660,19 @@ static hipError_t launch_mixed_t(const StageParams& P0, const HostOps& h, hipStr
     StageParams P = P0;
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
     P.nblk = ntx * nty;
+    static int resident = -1, ncu = 0;   // per kernel variant; the library drives one device
+    if (resident < 0) {
+        int dev = 0, r = 0;
+        hipDeviceProp_t prop{};
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, stage_mixed_kernel<NP, FLUX, CAPT, TX, TY>,
+                                                         TX * TY * NP, 0) != hipSuccess)
+            r = 0;
+        ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+        resident = r;
+    }
+    P.resident = resident;
+    P.ncu = ncu;
     hipLaunchKernelGGL((stage_mixed_kernel<NP, FLUX, CAPT, TX, TY>), dim3(P.nblk), dim3(TX * TY * NP), 0, s,
                        P, make_ops_mixed<NP>(h));
     return hipGetLastError();

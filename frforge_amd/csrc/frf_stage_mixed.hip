@@ -40,6 +40,13 @@
 #ifndef FRF_STAGGER
 #define FRF_STAGGER 1   // start delay step of the first resident blocks, in s_sleep 127 units
 #endif
+#ifndef FRF_MFMA
+// Line contractions on the matrix cores where Np <= 4 (sweep_mf).  Off: measured on MI355X at
+// 1024^2, p = 3, the 4x4x1 MFMA form is 15 % (null) and 4 % (Persson+BR0) slower than the VALU
+// form -- its accumulators raise register use and cost a wave per SIMD of occupancy, and the
+// kernel is bound by memory latency rather than by VALU issue.
+#define FRF_MFMA 0
+#endif
 #ifndef FRF_MWAVES
 #define FRF_MWAVES 1   // minimum waves per SIMD asked of the register allocator
 #endif
@@ -117,6 +124,95 @@ __device__ __forceinline__ void sweep(const OpsM<NP>& op, const float u[NP][kNeq
     line_deriv<NP>(op.D, H, 1.0f, V);
 }
 
+// ---- Line contractions on the matrix cores (Np <= 4).
+// v_mfma_f32_4x4x1_16b_f32 accumulates C += a b^T in each block of four lanes, lane r of a
+// block supplying row r of the column a and every lane its own b.  Over the K = Np steps
+// k, with lane r holding A[r][k] and every lane its own line value v[k], lane l ends with
+// C_l[r] = sum_k A[r][k] v_l[k]: the 4 x Np matrix A applied to each lane's own line, for all 64
+// lines of a wave at once and at twice the VALU FP32 rate.  The A operands are per-lane
+// constants (row lane % 4 of each matrix).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct LineMats {
+    float T1[NP];   // rows lL, lR, dL, dR: u traces and the face values of D u
+    float Dm[NP];   // rows of D
+    float T2[NP];   // rows lL, lR, 0, 0: flux traces
+    __device__ __forceinline__ void init(const OpsM<NP>& op) {
+        const int r = threadIdx.x & 3;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            T1[k] = r == 0 ? op.lL[k] : (r == 1 ? op.lR[k] : (r == 2 ? op.dL[k] : op.dR[k]));
+            T2[k] = r == 0 ? op.lL[k] : (r == 1 ? op.lR[k] : 0.0f);
+            float d = 0.0f;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) d = (r == m) ? op.D[m * NP + k] : d;
+            Dm[k] = d;
+        }
+    }
+};
+
+// C = A v_c for component c of the lane's line v
+template <int NP>
+__device__ __forceinline__ f4 line_mv(const float a[NP], const float v[NP][kNeq], int c) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < NP; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[k], v[k][c], acc, 0, 0, 0);
+    return acc;
+}
+
+// The sweep of `sweep` with its contractions on the matrix cores; must run on all lanes.
+template <int NP, int CAPT, int DIR>
+__device__ __forceinline__ void sweep_mf(const LineMats<NP>& A, const float u[NP][kNeq], float gamma,
+                                         float s, float V[NP][kNeq], float tHL[kNeq],
+                                         float tHR[kNeq], float uL[kNeq], float uR[kNeq],
+                                         float gL[kNeq], float gR[kNeq]) {
+    static_assert(NP <= 4, "one row group");
+    float H[NP][kNeq];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const Prim<float> q = primitives(u[k], gamma);
+        axis_flux<DIR>(u[k], q, H[k]);
+    }
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        const f4 tr = line_mv<NP>(A.T1, u, c);
+        uL[c] = tr[0];
+        uR[c] = tr[1];
+        if constexpr (CAPT == 1) {
+            gL[c] = s * tr[2];
+            gR[c] = s * tr[3];
+        }
+    }
+    if constexpr (CAPT != 0) {
+        if constexpr (CAPT == 2) {   // corrections of the DD form use the traces of F
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                const f4 tr = line_mv<NP>(A.T2, H, c);
+                tHL[c] = tr[0];
+                tHR[c] = tr[1];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            const f4 du = line_mv<NP>(A.Dm, u, c);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) H[k][c] -= s * du[k];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        if constexpr (CAPT != 2) {
+            const f4 tr = line_mv<NP>(A.T2, H, c);
+            tHL[c] = tr[0];
+            tHR[c] = tr[1];
+        }
+        const f4 v = line_mv<NP>(A.Dm, H, c);
+#pragma unroll
+        for (int m = 0; m < NP; ++m) V[m][c] = v[m];
+    }
+}
+
 // Face flux H^ = F^ - g^ between the minus (west/south) trace um and the plus trace up.
 template <int FLUX, int DIR, bool BR0>
 __device__ __forceinline__ void face_h(const float um[kNeq], const float up[kNeq],
@@ -158,6 +254,8 @@ __global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES) stage_mixed_kernel(Sta
     constexpr int FS = NT * NP;                 // component stride of the per-element face arrays
     constexpr int HX = TY * NP, HY = TX * NP;   // ... of the tile-edge (W/E, S/N) arrays
     constexpr int NW = (NTHR + 63) / 64;
+    // contractions on the matrix cores for Np <= 4 (one 4-row group per matrix)
+    constexpr bool MF = FRF_MFMA && NP <= 4 && NTHR % 64 == 0;
 
     __shared__ float sU[NT * ESTR];             // rows of u^(k); eta terms per column; scratch
     __shared__ float sXW[kNeq * FS];            // W-face u traces, then E-face fluxes H^
@@ -312,23 +410,31 @@ __global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES) stage_mixed_kernel(Sta
             }
         }
     }
-    // xi sweep of the own row, straight from the registers
+    // xi sweep of the own row, straight from the registers (on every lane when the matrix
+    // cores do the contractions: invalid lanes sweep the zero rows they stored)
     float rowV[NP][kNeq];
     float HW[kNeq], HE[kNeq], uE[kNeq], gE[kNeq];
-    if (valid) {
+    LineMats<MF ? NP : 1> mats;
+    if constexpr (MF) mats.init(op);
+    if (MF || valid) {
         float uW[kNeq], gW[kNeq];
-        sweep<NP, CAPT, 0>(op, ur, g, eps_e * iJx, rowV, HW, HE, uW, uE, gW, gE);
+        if constexpr (MF)
+            sweep_mf<NP, CAPT, 0>(mats, ur, g, eps_e * iJx, rowV, HW, HE, uW, uE, gW, gE);
+        else
+            sweep<NP, CAPT, 0>(op, ur, g, eps_e * iJx, rowV, HW, HE, uW, uE, gW, gE);
+        if (valid) {
 #pragma unroll
-        for (int c = 0; c < kNeq; ++c) {
-            sXW[c * FS + le * NP + t] = uW[c];
-            if constexpr (BR0) sXG[c * FS + le * NP + t] = gW[c];
+            for (int c = 0; c < kNeq; ++c) {
+                sXW[c * FS + le * NP + t] = uW[c];
+                if constexpr (BR0) sXG[c * FS + le * NP + t] = gW[c];
+            }
         }
     }
     __syncthreads();
 
     // ------------------------------------------------------------------ P1: eta sweep
     float HS[kNeq], HN[kNeq], uN[kNeq], gN[kNeq];
-    if (valid) {
+    if (MF || valid) {
         float* col = sU + le * ESTR + t;   // column i = t: [c][j] at c*NPP + j*NP
         float uc[NP][kNeq];
 #pragma unroll
@@ -336,16 +442,21 @@ __global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES) stage_mixed_kernel(Sta
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) uc[j][c] = col[c * NPP + j * NP];
         float V[NP][kNeq], uS[kNeq], gS[kNeq];
-        sweep<NP, CAPT, 1>(op, uc, g, eps_e * iJy, V, HS, HN, uS, uN, gS, gN);
+        if constexpr (MF)
+            sweep_mf<NP, CAPT, 1>(mats, uc, g, eps_e * iJy, V, HS, HN, uS, uN, gS, gN);
+        else
+            sweep<NP, CAPT, 1>(op, uc, g, eps_e * iJy, V, HS, HN, uS, uN, gS, gN);
+        if (valid) {
 #pragma unroll
-        for (int c = 0; c < kNeq; ++c) {
-            sYS[c * FS + le * NP + t] = uS[c];
-            if constexpr (BR0) sYG[c * FS + le * NP + t] = gS[c];
+            for (int c = 0; c < kNeq; ++c) {
+                sYS[c * FS + le * NP + t] = uS[c];
+                if constexpr (BR0) sYG[c * FS + le * NP + t] = gS[c];
+            }
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) col[c * NPP + j * NP] = V[j][c];
         }
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-#pragma unroll
-            for (int c = 0; c < kNeq; ++c) col[c * NPP + j * NP] = V[j][c];
     }
     __syncthreads();
 

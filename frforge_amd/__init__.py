@@ -12,7 +12,7 @@ from .equations import (Euler2D, conserved_to_primitives, pressure, primitives_t
 from .mesh import Mesh2D, PeriodicBC, TransmissiveBC, build_mesh_metrics
 from .operators import FROperators, build_operators
 from .solver import (compute_dt, element_viscosities, integrate, residual, sense, ssp_rk3,
-                     ssp_rk3_step)
+                     ssp_rk3_batch, ssp_rk3_step)
 from .state import (SchemeConfig, SolutionState2D, allocate_state, discrete_mass, has_nonfinite,
                     l2_error, positivity_ok, set_initial_condition)
 

@@ -20,7 +20,8 @@ OUT = os.path.join(HERE, "libfrforge_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 ARCH = os.environ.get("FRF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["frf_stage.hip", "frf_stage_mixed.hip", "frf_kernels.hip", "frf_capi.hip"]
+SOURCES = ["frf_stage.hip", "frf_stage_mixed.hip", "frf_batch_kernels.hip", "frf_kernels.hip",
+           "frf_capi.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

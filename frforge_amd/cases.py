@@ -100,3 +100,26 @@ def seeded_quadrant_states(seed: int) -> dict:
         p = g.uniform(0.1, 1.5)
         out[key] = (rho, u, v, p)
     return out
+
+
+ENSEMBLE_MESHES = (64, 128, 256, 512)
+ENSEMBLE_DEGREES = (2, 3, 4, 5)
+ENSEMBLE_FLUXES = ("rusanov", "hllc")
+
+
+def ensemble_instances(seed: int, n: int = 64, meshes=ENSEMBLE_MESHES, degrees=ENSEMBLE_DEGREES):
+    """The seeded ensemble of independent 2D instances: instance k uses seed + k; even k is an
+    isentropic vortex (periodic [-5,5]^2, NullCapturing), odd k a four-quadrant Riemann problem
+    with quadrant states from seeded_quadrant_states(seed + k) (transmissive [0,1]^2, Persson +
+    BR0).  Mesh size, degree and flux are drawn from SplitMix64(seed + k) in that order.
+    Returns a list of dicts {k, seed, kind, n, p, flux}."""
+    out = []
+    for k in range(n):
+        s = seed + k
+        g = SplitMix64(s ^ 0x5DEECE66D)
+        m = meshes[g.next_u64() % len(meshes)]
+        p = degrees[g.next_u64() % len(degrees)]
+        flux = ENSEMBLE_FLUXES[g.next_u64() % len(ENSEMBLE_FLUXES)]
+        out.append({"k": k, "seed": s, "kind": "vortex" if k % 2 == 0 else "riemann", "n": m, "p": p,
+                    "flux": flux})
+    return out

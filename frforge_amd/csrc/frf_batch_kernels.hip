@@ -1,0 +1,135 @@
+// Kernels of the batched form: many independent instances (each its own mesh, state and step
+// control) advanced together.  Instances that share degree, flux and capturing method form a
+// group whose stage is ONE launch: block b of the launch finds its instance by a binary search
+// over the group's block offsets and runs the FP32-mixed stage body (frf_mixed.h) on that
+// instance's tile.  Step control for all instances is one launch each for dt and finish.
+#include <hip/hip_runtime.h>
+
+#include "frf_launch.h"
+#include "frf_mixed.h"
+
+namespace frf {
+
+// Second tile shape per degree, the autotuner's alternative to TileM.
+template <int NP> struct TileAlt { static constexpr int TX = 8, TY = 8; };
+template <> struct TileAlt<4> { static constexpr int TX = 16, TY = 4; };
+template <> struct TileAlt<6> { static constexpr int TX = 4, TY = 8; };
+
+template <int NP, int V> struct TileOf { static constexpr int TX = TileM<NP>::TX, TY = TileM<NP>::TY; };
+template <int NP> struct TileOf<NP, 1> { static constexpr int TX = TileAlt<NP>::TX, TY = TileAlt<NP>::TY; };
+
+// Instance i of the group owns blocks [start[i], start[i+1]); every start is a multiple of 8 so
+// that the XCD remap of an instance's tiles (xcd_remap) sees block ids with the hardware's
+// round-robin phase.  Blocks past an instance's own tile count are padding and exit.
+template <int NP, int FLUX, int CAPT, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES)
+    stage_mixed_group_kernel(const StageParams* __restrict__ Ps, const int* __restrict__ start, int n,
+                             OpsM<NP> op) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n;   // invariant: start[lo] <= b < start[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const StageParams& P = Ps[lo];
+    const int bid = b - start[lo];
+    if (bid >= P.nblk) return;
+    stage_mixed_body<NP, FLUX, CAPT, TX, TY>(P, op, bid);
+}
+
+void mixed_tile(int np, int variant, int* tx, int* ty) {
+    switch (np * 2 + (variant ? 1 : 0)) {
+#define FRF_TILE_CASE(N)                                                  \
+    case 2 * N: *tx = TileOf<N, 0>::TX; *ty = TileOf<N, 0>::TY; return;   \
+    case 2 * N + 1: *tx = TileOf<N, 1>::TX; *ty = TileOf<N, 1>::TY; return;
+        FRF_TILE_CASE(2)
+        FRF_TILE_CASE(3)
+        FRF_TILE_CASE(4)
+        FRF_TILE_CASE(5)
+        FRF_TILE_CASE(6)
+#undef FRF_TILE_CASE
+        default: *tx = *ty = 0;
+    }
+}
+
+template <int NP, int FLUX, int CAPT, int V>
+static hipError_t group_t(const StageParams* Ps, const int* start, int n, int nblocks, const HostOps& h,
+                          hipStream_t s) {
+    constexpr int TX = TileOf<NP, V>::TX, TY = TileOf<NP, V>::TY;
+    hipLaunchKernelGGL((stage_mixed_group_kernel<NP, FLUX, CAPT, TX, TY>), dim3(nblocks), dim3(TX * TY * NP),
+                       0, s, Ps, start, n, make_ops_mixed<NP>(h));
+    return hipGetLastError();
+}
+
+template <int NP, int V>
+static hipError_t group_fc(int flux, int capt, const StageParams* Ps, const int* start, int n,
+                           int nblocks, const HostOps& h, hipStream_t s) {
+    if (flux == 0) {
+        if (capt == 0) return group_t<NP, 0, 0, V>(Ps, start, n, nblocks, h, s);
+        if (capt == 1) return group_t<NP, 0, 1, V>(Ps, start, n, nblocks, h, s);
+        return group_t<NP, 0, 2, V>(Ps, start, n, nblocks, h, s);
+    }
+    if (capt == 0) return group_t<NP, 1, 0, V>(Ps, start, n, nblocks, h, s);
+    if (capt == 1) return group_t<NP, 1, 1, V>(Ps, start, n, nblocks, h, s);
+    return group_t<NP, 1, 2, V>(Ps, start, n, nblocks, h, s);
+}
+
+hipError_t launch_stage_mixed_group(int np, int variant, int flux, int capt, const StageParams* Ps,
+                                    const int* start, int n, int nblocks, const HostOps& h,
+                                    hipStream_t s) {
+    switch (np * 2 + (variant ? 1 : 0)) {
+#define FRF_GROUP_CASE(N)                                                     \
+    case 2 * N: return group_fc<N, 0>(flux, capt, Ps, start, n, nblocks, h, s); \
+    case 2 * N + 1: return group_fc<N, 1>(flux, capt, Ps, start, n, nblocks, h, s);
+        FRF_GROUP_CASE(2)
+        FRF_GROUP_CASE(3)
+        FRF_GROUP_CASE(4)
+        FRF_GROUP_CASE(5)
+        FRF_GROUP_CASE(6)
+#undef FRF_GROUP_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------- step control over instances
+
+__global__ void dt_group_kernel(DevCtl* const* __restrict__ c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dt_step(c[i]);
+}
+
+__global__ void finish_group_kernel(DevCtl* const* __restrict__ c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) finish_step(c[i]);
+}
+
+// out[0] = number of instances whose run has not finished
+__global__ void pending_kernel(DevCtl* const* __restrict__ c, int n, int* out) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (c[i]->done == 0) atomicAdd(&cnt, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = cnt;
+}
+
+hipError_t launch_dt_group(DevCtl* const* c, int n, hipStream_t s) {
+    hipLaunchKernelGGL(dt_group_kernel, dim3((n + 63) / 64), dim3(64), 0, s, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_group(DevCtl* const* c, int n, hipStream_t s) {
+    hipLaunchKernelGGL(finish_group_kernel, dim3((n + 63) / 64), dim3(64), 0, s, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pending(DevCtl* const* c, int n, int* out, hipStream_t s) {
+    hipLaunchKernelGGL(pending_kernel, dim3(1), dim3(256), 0, s, c, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace frf

@@ -656,3 +656,324 @@ extern "C" int32_t frf_device_count(int32_t* n) {
 extern "C" int32_t frf_device_synchronize(void) {
     return hipDeviceSynchronize() == hipSuccess ? FRF_OK : FRF_E_HIP;
 }
+
+// ---------------------------------------------------------------- batched form
+
+struct frf_batch {
+    struct Group {
+        int np = 0, flux = 0, capt = 0, variant = 0;
+        std::vector<int> members;                       // instance indices
+        std::vector<int> start;                         // first block of each member, + total
+        frf::StageParams* dP[3] = {nullptr, nullptr, nullptr};   // per stage, one per member
+        int* dStart = nullptr;
+    };
+    std::vector<frf_ctx*> inst;
+    std::vector<Group> groups;
+    hipStream_t stream = nullptr;
+    DevCtl** dCtls = nullptr;
+    int* dPending = nullptr;
+    int* hPending = nullptr;                            // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool layout_dirty = true;                           // tile variants changed: re-capture
+    std::string err;
+};
+
+static constexpr int kBatchGraphSteps = 8;   // SSP-RK3 steps per graph replay
+
+static int32_t bfail(frf_batch* b, int32_t code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+#define BHIPCHK(b, expr)                                                                     \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess) return bfail((b), FRF_E_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+extern "C" const char* frf_batch_last_error(const frf_batch* b) { return b ? b->err.c_str() : ""; }
+
+extern "C" int32_t frf_batch_destroy(frf_batch* b) {
+    if (!b) return FRF_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& g : b->groups) {
+        for (int k = 0; k < 3; ++k) if (g.dP[k]) (void)hipFree(g.dP[k]);
+        if (g.dStart) (void)hipFree(g.dStart);
+    }
+    if (b->dCtls) (void)hipFree(b->dCtls);
+    if (b->dPending) (void)hipFree(b->dPending);
+    if (b->hPending) (void)hipHostFree(b->hPending);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch** out) {
+    if (!out) return FRF_E_ARG;
+    *out = nullptr;
+    if (!ctxs || n < 1) {
+        g_create_err = "frf_batch_create: need at least one context";
+        return FRF_E_ARG;
+    }
+    for (int i = 0; i < n; ++i) {
+        frf_ctx* c = ctxs[i];
+        if (!c || !c->ops_set || !c->geom_set || !c->state_set) {
+            g_create_err = "frf_batch_create: instance " + std::to_string(i) + " is not ready (operators, geometry, state)";
+            return c ? FRF_E_STATE : FRF_E_ARG;
+        }
+        if (c->pb.precision != FRF_FP32_MIXED) {
+            g_create_err = "frf_batch_create: the batched form runs the FP32-mixed form only";
+            return FRF_E_UNSUPPORTED;
+        }
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == c) {
+                g_create_err = "frf_batch_create: a context appears twice";
+                return FRF_E_ARG;
+            }
+    }
+    frf_batch* b = new frf_batch();
+    b->inst.assign(ctxs, ctxs + n);
+    for (int i = 0; i < n; ++i) {
+        const frf_ctx* c = ctxs[i];
+        frf_batch::Group* g = nullptr;
+        for (auto& gg : b->groups)
+            if (gg.np == c->np && gg.flux == c->pb.flux && gg.capt == c->pb.capturing) g = &gg;
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->np = c->np;
+            g->flux = c->pb.flux;
+            g->capt = c->pb.capturing;
+        }
+        g->members.push_back(i);
+    }
+    std::vector<DevCtl*> ctl(n);
+    for (int i = 0; i < n; ++i) ctl[i] = ctxs[i]->dCtl;
+    auto cleanup = [&](const char* what) {
+        g_create_err = std::string("frf_batch_create: ") + what;
+        frf_batch_destroy(b);
+        return FRF_E_HIP;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+    if (hipMalloc(&b->dCtls, n * sizeof(DevCtl*)) != hipSuccess ||
+        hipMalloc(&b->dPending, sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&b->hPending, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return cleanup("allocation");
+    if (hipMemcpy(b->dCtls, ctl.data(), n * sizeof(DevCtl*), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup("copy");
+    for (auto& g : b->groups) {
+        const size_t m = g.members.size();
+        for (int k = 0; k < 3; ++k)
+            if (hipMalloc(&g.dP[k], m * sizeof(frf::StageParams)) != hipSuccess) return cleanup("allocation");
+        if (hipMalloc(&g.dStart, (m + 1) * sizeof(int)) != hipSuccess) return cleanup("allocation");
+    }
+    *out = b;
+    return FRF_OK;
+}
+
+// Stage parameters of every member of group g for its current tile variant, and the block
+// offsets; want_lambda[i]: instance i steps with a CFL dt (stage 3 accumulates its wave speed).
+static int32_t batch_upload_group(frf_batch* b, frf_batch::Group& g, const std::vector<char>& want_lambda) {
+    int tx = 0, ty = 0;
+    frf::mixed_tile(g.np, g.variant, &tx, &ty);
+    const size_t m = g.members.size();
+    std::vector<frf::StageParams> P[3];
+    g.start.assign(m + 1, 0);
+    for (size_t k = 0; k < m; ++k) {
+        frf_ctx* c = b->inst[g.members[k]];
+        double *U0 = c->dU[0], *U1 = c->dU[1], *U2 = c->dU[2];
+        double *E0 = c->dEps[0], *E1 = c->dEps[1], *E2 = c->dEps[2];
+        frf::StageParams S[3] = {stage_params(c, U0, U0, U1, 1, E0, E1, 0),
+                                 stage_params(c, U1, U0, U2, 2, E1, E2, 0),
+                                 stage_params(c, U2, U0, U0, 3, E2, E0, want_lambda[g.members[k]] ? 1 : 0)};
+        const int nblk = ((c->pb.nx + tx - 1) / tx) * ((c->pb.ny + ty - 1) / ty);
+        for (int s = 0; s < 3; ++s) {
+            S[s].nblk = nblk;
+            S[s].resident = 0;   // no start staggering inside a grouped launch
+            S[s].ncu = 1;
+            P[s].push_back(S[s]);
+        }
+        g.start[k + 1] = g.start[k] + ((nblk + 7) / 8) * 8;
+    }
+    for (int s = 0; s < 3; ++s)
+        BHIPCHK(b, hipMemcpyAsync(g.dP[s], P[s].data(), m * sizeof(frf::StageParams), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipMemcpyAsync(g.dStart, g.start.data(), (m + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return FRF_OK;
+}
+
+static int32_t batch_enqueue_stage(frf_batch* b, frf_batch::Group& g, int stage) {
+    const frf_ctx* c0 = b->inst[g.members[0]];
+    BHIPCHK(b, frf::launch_stage_mixed_group(g.np, g.variant, g.flux, g.capt, g.dP[stage - 1], g.dStart,
+                                             (int)g.members.size(), g.start.back(), c0->ops, b->stream));
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_batch_tile(const frf_batch* b, int32_t i, int32_t* tx, int32_t* ty) {
+    if (!b || !tx || !ty || i < 0 || i >= (int32_t)b->inst.size()) return FRF_E_ARG;
+    for (const auto& g : b->groups)
+        for (int m : g.members)
+            if (m == i) {
+                int x = 0, y = 0;
+                frf::mixed_tile(g.np, g.variant, &x, &y);
+                *tx = x;
+                *ty = y;
+                return FRF_OK;
+            }
+    return FRF_E_ARG;
+}
+
+extern "C" int32_t frf_batch_tune(frf_batch* b, int32_t repeats) {
+    if (!b) return FRF_E_ARG;
+    if (repeats < 1 || repeats > 100) return bfail(b, FRF_E_ARG, "repeats must be in 1..100");
+    const int n = (int)b->inst.size();
+    for (int i = 0; i < n; ++i) {   // dt = 0 in every step-control block: stage 1 copies u^n
+        frf_ctx* c = b->inst[i];
+        if (settle(c)) return bfail(b, FRF_E_HIP, "instance %d: %s", i, c->err.c_str());
+        init_ctl(c, INFINITY, 0.0, 0.0, 1);
+        if (hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return bfail(b, FRF_E_HIP, "instance %d: step-control upload failed", i);
+    }
+    const std::vector<char> no_lambda(n, 0);
+    hipEvent_t e0, e1;
+    BHIPCHK(b, hipEventCreate(&e0));
+    BHIPCHK(b, hipEventCreate(&e1));
+    int32_t r = FRF_OK;
+    for (auto& g : b->groups) {
+        float best = 0.f;
+        int best_v = 0;
+        for (int v = 0; v < 2 && r == FRF_OK; ++v) {
+            g.variant = v;
+            if ((r = batch_upload_group(b, g, no_lambda))) break;
+            if ((r = batch_enqueue_stage(b, g, 1))) break;   // warm-up
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { r = bfail(b, FRF_E_HIP, "event"); break; }
+            for (int k = 0; k < repeats && r == FRF_OK; ++k) r = batch_enqueue_stage(b, g, 1);
+            if (r) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                r = bfail(b, FRF_E_HIP, "event timing");
+                break;
+            }
+            if (v == 0 || ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        g.variant = best_v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->layout_dirty = true;
+    return r;
+}
+
+// Capture kBatchGraphSteps steps of every instance (dt, three stages per group, finish).
+static int32_t batch_capture(frf_batch* b) {
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    const int n = (int)b->inst.size();
+    BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int32_t r = FRF_OK;
+    for (int s = 0; s < kBatchGraphSteps && r == FRF_OK; ++s) {
+        if (frf::launch_dt_group(b->dCtls, n, b->stream) != hipSuccess) r = bfail(b, FRF_E_HIP, "dt launch");
+        for (int stage = 1; stage <= 3 && r == FRF_OK; ++stage)
+            for (auto& g : b->groups)
+                if (r == FRF_OK) r = batch_enqueue_stage(b, g, stage);
+        if (r == FRF_OK && frf::launch_finish_group(b->dCtls, n, b->stream) != hipSuccess)
+            r = bfail(b, FRF_E_HIP, "finish launch");
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (r) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (e != hipSuccess) return bfail(b, FRF_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = graph;
+    BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->layout_dirty = false;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_batch_run(frf_batch* b, const double* t_final, const double* cfl,
+                                 const double* fixed_dt, const int64_t* max_steps,
+                                 frf_run_result* results, double* elapsed_ms) {
+    if (!b) return FRF_E_ARG;
+    if (!t_final || !cfl || !fixed_dt || !max_steps || !results)
+        return bfail(b, FRF_E_ARG, "null argument array");
+    const int n = (int)b->inst.size();
+    for (int i = 0; i < n; ++i) {
+        if (max_steps[i] < 1) return bfail(b, FRF_E_ARG, "instance %d: max_steps must be >= 1", i);
+        if (!(fixed_dt[i] > 0.0) && !(cfl[i] > 0.0))
+            return bfail(b, FRF_E_ARG, "instance %d: cfl must be > 0 for CFL steps", i);
+    }
+    std::vector<char> adaptive(n);
+    for (int i = 0; i < n; ++i) {
+        frf_ctx* c = b->inst[i];
+        adaptive[i] = !(fixed_dt[i] > 0.0);
+        if (settle(c)) return bfail(b, FRF_E_HIP, "instance %d: %s", i, c->err.c_str());
+        init_ctl(c, t_final[i], cfl[i], fixed_dt[i] > 0.0 ? fixed_dt[i] : 0.0, max_steps[i]);
+        if (hipMemcpyAsync(c->dCtl, c->hCtl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            enqueue_prepass(c, adaptive[i] != 0) != FRF_OK || hipStreamSynchronize(c->stream) != hipSuccess)
+            return bfail(b, FRF_E_HIP, "instance %d: set-up failed: %s", i, c->err.c_str());
+    }
+    for (auto& g : b->groups) {
+        int32_t r = batch_upload_group(b, g, adaptive);
+        if (r) return r;
+    }
+    if (b->layout_dirty || !b->exec) {
+        int32_t r = batch_capture(b);
+        if (r) return r;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIPCHK(b, hipEventCreate(&e0));
+    BHIPCHK(b, hipEventCreate(&e1));
+    int32_t r = FRF_OK;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) r = bfail(b, FRF_E_HIP, "event");
+    while (r == FRF_OK) {
+        if (hipGraphLaunch(b->exec, b->stream) != hipSuccess ||
+            frf::launch_pending(b->dCtls, n, b->dPending, b->stream) != hipSuccess ||
+            hipMemcpyAsync(b->hPending, b->dPending, sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess) {
+            r = bfail(b, FRF_E_HIP, "step loop failed");
+            break;
+        }
+        if (*b->hPending == 0) break;
+    }
+    float ms = 0.f;
+    if (r == FRF_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        r = bfail(b, FRF_E_HIP, "event timing");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    if (elapsed_ms) *elapsed_ms = ms;
+    for (int i = 0; i < n; ++i) {
+        frf_ctx* c = b->inst[i];
+        if (read_ctl(c)) return bfail(b, FRF_E_HIP, "instance %d: %s", i, c->err.c_str());
+        const DevCtl& h = *c->hCtl;
+        c->t = h.t;
+        const int fs = h.fail_stage;
+        c->cur = fs < kNoFail ? observable_buffer(fs) : 0;
+        results[i].status = h.status;
+        results[i].fail_stage = fs < kNoFail ? fs : 0;
+        results[i].n_steps = h.n_steps;
+        results[i].t = h.t;
+    }
+    return FRF_OK;
+}

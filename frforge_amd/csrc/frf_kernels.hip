@@ -120,45 +120,8 @@ __global__ void __launch_bounds__(EPB* NP) elem_kernel(ElemParams P, OpsT<NP, do
     }
 }
 
-// Step start: loop test, CFL or fixed step, last-step truncation (SSP_RK3.jl:185-189, 126-133).
-__global__ void dt_kernel(DevCtl* c) {
-    if (c->done != 0) return;
-    const double t = c->t;
-    if (!(t < c->t_final - 10.0 * kEps64)) {
-        c->done = 1;
-        return;
-    }
-    double step;
-    if (c->fixed_dt > 0.0) {
-        step = c->fixed_dt;
-    } else {
-        double lam = __longlong_as_double((long long)c->lam_bits);
-        lam = (lam > kEps64 || lam != lam) ? lam : kEps64;
-        step = c->cfl * c->hmin / (double(2 * c->p + 1) * lam);
-    }
-    c->lam_bits = 0ull;
-    if (t + step > c->t_final) step = c->t_final - t;
-    c->dt = step;
-    c->t0 = t;
-}
-
-// Step end: step count, state time as the reference leaves it, status (SSP_RK3.jl:190-208).
-__global__ void finish_kernel(DevCtl* c) {
-    if (c->done != 0) return;
-    c->n_steps += 1;
-    if (c->fail_stage < kNoFail) {
-        const int fs = c->fail_stage;
-        c->status = 1;
-        c->t = (fs == 1) ? c->t0 : (fs == 3 ? c->t0 + c->dt / 2.0 : c->t0 + c->dt);
-        c->done = 1;
-        return;
-    }
-    c->t = c->t0 + c->dt;
-    if (c->n_steps >= c->max_steps) {
-        c->status = 2;
-        c->done = 1;
-    }
-}
+__global__ void dt_kernel(DevCtl* c) { dt_step(c); }
+__global__ void finish_kernel(DevCtl* c) { finish_step(c); }
 
 // ---------------------------------------------------------------- host-side launchers
 

@@ -24,6 +24,17 @@ hipError_t launch_stage_mixed(int np, int flux, int capt, const StageParams& P, 
 hipError_t launch_elem(int np, int prec, const ElemParams& P, const HostOps& h, int accum_lambda,
                        int check_ctl, hipStream_t s);
 hipError_t launch_dt(DevCtl* c, hipStream_t s);
+
+// Batched form (frf_batch_kernels.hip): one launch per group of instances sharing degree, flux
+// and capturing; Ps and start in device memory, start[i] the first block of instance i
+// (multiples of 8), nblocks = start[n].  variant selects the tile shape (mixed_tile).
+void mixed_tile(int np, int variant, int* tx, int* ty);
+hipError_t launch_stage_mixed_group(int np, int variant, int flux, int capt, const StageParams* Ps,
+                                    const int* start, int n, int nblocks, const HostOps& h,
+                                    hipStream_t s);
+hipError_t launch_dt_group(DevCtl* const* c, int n, hipStream_t s);
+hipError_t launch_finish_group(DevCtl* const* c, int n, hipStream_t s);
+hipError_t launch_pending(DevCtl* const* c, int n, int* out, hipStream_t s);
 hipError_t launch_finish(DevCtl* c, hipStream_t s);
 
 }  // namespace frf

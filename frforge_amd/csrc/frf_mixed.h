@@ -1,0 +1,770 @@
+// FP32-mixed stage kernel body for gfx950 (shared by the single-instance launch in
+// frf_stage_mixed.hip and the grouped batch launch in frf_batch.hip): one launch evaluates the FR residual of an SSP-RK3 stage in
+// FP32 and forms the stage state in FP64.
+//
+// On the affine meshes of the reference every element is a rectangle: x_xi = Jx, y_eta = Jy and
+// the cross metrics x_eta, y_xi vanish (the reference carries them as roundoff of D X, relative
+// size 1e-16, far below single precision).  The residual then separates by direction,
+//
+//   du = -(1/Jx) [ D H  + (H^_W - H_W) g'_L + (H^_E - H_E) g'_R ]_xi
+//        -(1/Jy) [ D H  + (H^_S - H_S) g'_L + (H^_N - H_N) g'_R ]_eta ,
+//
+// with H = F - g the inviscid minus the BR0 viscous flux of the direction (g = eps/J D u,
+// PerssonAV2D.jl:241-242), H_W etc. its Lagrange traces, and H^ = F^ - g^ the Riemann flux
+// minus the BR0 face flux (SharedAV.jl:29-42) on the face.  This is Residual2D.jl:336-398 plus
+// PerssonAV2D.jl:203-398 with the metric factors cancelled; one derivative, one pair of traces and
+// one correction per direction serve both the inviscid and the viscous part.  For the local-DD
+// form (PerssonAV2D.jl:135-194) the volume flux is F - (eps/J) D u and the corrections use F.
+//
+// Work split (Np line threads per element, a TX x TY element tile per workgroup):
+//   P0  each thread loads row j = t of u^(k) (and of u^n for stages 2 and 3) in FP64, keeps the
+//       FP64 stage-combine base, writes the row to LDS in FP32 and sweeps it at once in
+//       registers (xi direction: F, D u, H, D H, traces).  Spare threads reduce the tile's
+//       face-adjacent halo lines to traces.
+//   P1  thread t sweeps column i = t out of LDS (eta direction) and overwrites that column with
+//       its eta volume term; no other thread reads it.
+//   P2  each face is evaluated once, E faces by row threads and N faces by column threads; the
+//       tile's W and S edge faces by a compact set of task threads.
+//   P3  eta corrections in place in LDS, then the row threads form du, the FP64 combine, the
+//       non-finite flags and the store; stage 3 writes u^(n+1) over u^n.
+//   P4  epilogue on the new state: Persson sensor and element viscosity for the next stage and
+//       the maximum wave speed for the next CFL step, in FP32.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frf_device.h"
+#include "frf_kernels.h"
+#include "frf_launch.h"
+#include "frf_physics.h"
+
+#ifndef FRF_STAGGER
+#define FRF_STAGGER 1   // start delay step of the first resident blocks, in s_sleep 127 units
+#endif
+#ifndef FRF_MFMA
+// Line contractions on the matrix cores where Np <= 4 (sweep_mf).  Off: measured on MI355X at
+// 1024^2, p = 3, the 4x4x1 MFMA form is 15 % (null) and 4 % (Persson+BR0) slower than the VALU
+// form -- its accumulators raise register use and cost a wave per SIMD of occupancy, and the
+// kernel is bound by memory latency rather than by VALU issue.
+#define FRF_MFMA 0
+#endif
+#ifndef FRF_MWAVES
+#define FRF_MWAVES 1   // minimum waves per SIMD asked of the register allocator
+#endif
+
+namespace frf {
+
+// One line (ROW: row `line`, else column `line`) of a halo element reduced to its trace with
+// weights l, and for BR0 to the trace geps * (dl . v) of its viscous flux; cs = component stride.
+template <int NP, bool BR0, bool ROW>
+__device__ __forceinline__ void halo_line(const double* __restrict__ u, int nel, int en, int line,
+                                          const float* l, const float* dl, float geps, float* su,
+                                          float* sg, int cs) {
+    float v[NP][kNeq];
+    if constexpr (ROW)
+        load_row<NP>(u, nel, en, line, v);
+    else
+        load_col<NP>(u, nel, en, line, v);
+    float tr[kNeq];
+    trace<NP>(l, v, tr);
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) su[c * cs] = tr[c];
+    if constexpr (BR0) {
+        trace<NP>(dl, v, tr);
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) sg[c * cs] = geps * tr[c];
+    }
+}
+
+// BR0 interior penalty 2 eps_bar (p+1) / h_bar (SharedAV.jl:29-35) in FP32
+__device__ __forceinline__ float br0_tau_f(float em, float ep, float hm, float hp, int p) {
+    const float hbar = rmax(0.5f * (hm + hp), float(kEps64));
+    return (em + ep) * float(p + 1) * rcp_(hbar);
+}
+
+// Direction sweep of one line held in registers (DIR 0: a row, xi; 1: a column, eta).
+// Out: H = F - g along the line (F for no capturing), its derivative V = D H (of F - s D u for
+// DD), the traces tH at both ends (of H; of F for DD), the u traces and, for BR0, the g traces.
+template <int NP, int CAPT, int DIR>
+__device__ __forceinline__ void sweep(const OpsM<NP>& op, const float u[NP][kNeq], float gamma,
+                                      float s, float V[NP][kNeq], float tHL[kNeq],
+                                      float tHR[kNeq], float uL[kNeq], float uR[kNeq],
+                                      float gL[kNeq], float gR[kNeq]) {
+    float H[NP][kNeq];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const Prim<float> q = primitives(u[k], gamma);
+        axis_flux<DIR>(u[k], q, H[k]);
+    }
+    trace<NP>(op.lL, u, uL);
+    trace<NP>(op.lR, u, uR);
+    if constexpr (CAPT != 0) {
+        float Du[NP][kNeq];
+        line_deriv<NP>(op.D, u, 1.0f, Du);
+        if constexpr (CAPT == 2) {   // corrections of the DD form use the traces of F
+            trace<NP>(op.lL, H, tHL);
+            trace<NP>(op.lR, H, tHR);
+        } else {   // face values of D u as dL . u, exactly as the halo lines form them
+            trace<NP>(op.dL, u, gL);
+            trace<NP>(op.dR, u, gR);
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                gL[c] *= s;
+                gR[c] *= s;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) H[k][c] -= s * Du[k][c];
+    }
+    if constexpr (CAPT != 2) {
+        trace<NP>(op.lL, H, tHL);
+        trace<NP>(op.lR, H, tHR);
+    }
+    line_deriv<NP>(op.D, H, 1.0f, V);
+}
+
+// ---- Line contractions on the matrix cores (Np <= 4).
+// v_mfma_f32_4x4x1_16b_f32 accumulates C += a b^T in each block of four lanes, lane r of a
+// block supplying row r of the column a and every lane its own b.  Over the K = Np steps
+// k, with lane r holding A[r][k] and every lane its own line value v[k], lane l ends with
+// C_l[r] = sum_k A[r][k] v_l[k]: the 4 x Np matrix A applied to each lane's own line, for all 64
+// lines of a wave at once and at twice the VALU FP32 rate.  The A operands are per-lane
+// constants (row lane % 4 of each matrix).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct LineMats {
+    float T1[NP];   // rows lL, lR, dL, dR: u traces and the face values of D u
+    float Dm[NP];   // rows of D
+    float T2[NP];   // rows lL, lR, 0, 0: flux traces
+    __device__ __forceinline__ void init(const OpsM<NP>& op) {
+        const int r = threadIdx.x & 3;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            T1[k] = r == 0 ? op.lL[k] : (r == 1 ? op.lR[k] : (r == 2 ? op.dL[k] : op.dR[k]));
+            T2[k] = r == 0 ? op.lL[k] : (r == 1 ? op.lR[k] : 0.0f);
+            float d = 0.0f;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) d = (r == m) ? op.D[m * NP + k] : d;
+            Dm[k] = d;
+        }
+    }
+};
+
+// C = A v_c for component c of the lane's line v
+template <int NP>
+__device__ __forceinline__ f4 line_mv(const float a[NP], const float v[NP][kNeq], int c) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < NP; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[k], v[k][c], acc, 0, 0, 0);
+    return acc;
+}
+
+// The sweep of `sweep` with its contractions on the matrix cores; must run on all lanes.
+template <int NP, int CAPT, int DIR>
+__device__ __forceinline__ void sweep_mf(const LineMats<NP>& A, const float u[NP][kNeq], float gamma,
+                                         float s, float V[NP][kNeq], float tHL[kNeq],
+                                         float tHR[kNeq], float uL[kNeq], float uR[kNeq],
+                                         float gL[kNeq], float gR[kNeq]) {
+    static_assert(NP <= 4, "one row group");
+    float H[NP][kNeq];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const Prim<float> q = primitives(u[k], gamma);
+        axis_flux<DIR>(u[k], q, H[k]);
+    }
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        const f4 tr = line_mv<NP>(A.T1, u, c);
+        uL[c] = tr[0];
+        uR[c] = tr[1];
+        if constexpr (CAPT == 1) {
+            gL[c] = s * tr[2];
+            gR[c] = s * tr[3];
+        }
+    }
+    if constexpr (CAPT != 0) {
+        if constexpr (CAPT == 2) {   // corrections of the DD form use the traces of F
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                const f4 tr = line_mv<NP>(A.T2, H, c);
+                tHL[c] = tr[0];
+                tHR[c] = tr[1];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            const f4 du = line_mv<NP>(A.Dm, u, c);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) H[k][c] -= s * du[k];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        if constexpr (CAPT != 2) {
+            const f4 tr = line_mv<NP>(A.T2, H, c);
+            tHL[c] = tr[0];
+            tHR[c] = tr[1];
+        }
+        const f4 v = line_mv<NP>(A.Dm, H, c);
+#pragma unroll
+        for (int m = 0; m < NP; ++m) V[m][c] = v[m];
+    }
+}
+
+// Face flux H^ = F^ - g^ between the minus (west/south) trace um and the plus trace up.
+template <int FLUX, int DIR, bool BR0>
+__device__ __forceinline__ void face_h(const float um[kNeq], const float up[kNeq],
+                                       const float gm[kNeq], const float gp[kNeq], float tau,
+                                       float gamma, float out[kNeq]) {
+    iflux_axis<FLUX, DIR>(um, up, gamma, out);
+    if constexpr (BR0) {
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) out[c] -= 0.5f * (gm[c] + gp[c]) - tau * (up[c] - um[c]);
+    }
+}
+
+// Transmissive boundary face: ghost = interior, so F^ is the physical flux of the trace and
+// g^ the interior g trace (BoundaryConditions.jl:43-132, PerssonAV2D.jl boundary branch).
+template <int DIR, bool BR0>
+__device__ __forceinline__ void boundary_h(const float ub[kNeq], const float gb[kNeq], float gamma,
+                                           float out[kNeq]) {
+    const Prim<float> q = primitives(ub, gamma);
+    axis_flux<DIR>(ub, q, out);
+    if constexpr (BR0) {
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) out[c] -= gb[c];
+    }
+}
+
+template <int NP, int FLUX, int CAPT, int TX, int TY>
+__device__ __forceinline__ void stage_mixed_body(const StageParams& P, const OpsM<NP>& op, int bid) {
+    using L = TabLayout<NP>;
+    constexpr int CT = L::STRIDE;
+    constexpr int NT = TX * TY, NPP = NP * NP, NTHR = NT * NP;
+    constexpr bool BR0 = CAPT == 1, AV = CAPT != 0;
+    constexpr int BANKS = 64;
+    // element slot in sU, padded so that the Np lanes of the elements in a wave fall in different
+    // banks on the row writes and the column reads: stride = Np+1 (mod banks)
+    constexpr int ESTR = kNeq * NPP + ((NP + 1 - (kNeq * NPP) % BANKS) % BANKS + BANKS) % BANKS;
+    static_assert(ESTR % BANKS == (NP + 1) % BANKS, "sU stride");
+    static_assert(NPP + 3 * NP <= ESTR, "epilogue scratch fits an element slot");
+    static_assert(TX * TY >= 2 * (TX + TY), "one thread per halo line task");
+    constexpr int FS = NT * NP;                 // component stride of the per-element face arrays
+    constexpr int HX = TY * NP, HY = TX * NP;   // ... of the tile-edge (W/E, S/N) arrays
+    constexpr int NW = (NTHR + 63) / 64;
+    // contractions on the matrix cores for Np <= 4 (one 4-row group per matrix)
+    constexpr bool MF = FRF_MFMA && NP <= 4 && NTHR % 64 == 0;
+
+    __shared__ float sU[NT * ESTR];             // rows of u^(k); eta terms per column; scratch
+    __shared__ float sXW[kNeq * FS];            // W-face u traces, then E-face fluxes H^
+    __shared__ float sYS[kNeq * FS];            // S-face u traces, then N-face fluxes H^
+    __shared__ float sXG[BR0 ? kNeq * FS : 1];  // BR0: W-face g traces
+    __shared__ float sYG[BR0 ? kNeq * FS : 1];  //      S-face g traces
+    __shared__ float sHW[kNeq * HX], sHE[kNeq * HX];   // halo u traces: W halo east, E halo west
+    __shared__ float sHS[kNeq * HY], sHN[kNeq * HY];   //                S halo north, N halo south
+    __shared__ float sGW[BR0 ? kNeq * HX : 1], sGE[BR0 ? kNeq * HX : 1];
+    __shared__ float sGS[BR0 ? kNeq * HY : 1], sGN[BR0 ? kNeq * HY : 1];
+    __shared__ float sF0x[kNeq * HX], sF0y[kNeq * HY];   // H^ on the tile's W and S edges
+    __shared__ float sEps[AV ? NT : 1];
+    __shared__ float sEpsH[BR0 ? 2 * (TX + TY) : 1];     // halo viscosities: W, E, S, N
+    __shared__ float sWave[NW];
+
+    if (P.stage > 0) {
+        // Skip after a failure in an earlier stage of this step: reads only values written by
+        // earlier launches (this stage reports P.stage or 4, never less), so it is uniform.
+        if (P.ctl->done != 0 || P.ctl->fail_stage < P.stage) return;
+    }
+    if constexpr (FRF_STAGGER > 0) {
+        // The first resident blocks of a CU start together and would run their load and their
+        // compute phases in lockstep.  Blocks are dispatched round-robin over the CUs, so block b
+        // of the first P.ncu * P.resident is the (b / ncu)-th of its CU: delay it by that many
+        // steps, so that the loads of some blocks overlap the compute of others from the start.
+        const int slot = bid / P.ncu;
+        if (slot < P.resident)
+            for (int k = 0; k < slot * FRF_STAGGER; ++k) __builtin_amdgcn_s_sleep(127);
+    }
+    const int nx = P.nx, ny = P.ny, nel = P.nel;
+    const int ntx = (nx + TX - 1) / TX;
+    const int blk = xcd_remap(bid, P.nblk);
+    const int nty = (ny + TY - 1) / TY;
+    int btx, bty;
+    band_tile(blk, ntx, nty, FRF_BANDW, btx, bty);
+    const int tx0 = btx * TX, ty0 = bty * TY;
+    const int nvx = (nx - tx0 < TX) ? nx - tx0 : TX;
+    const int nvy = (ny - ty0 < TY) ? ny - ty0 : TY;
+    const int tid = threadIdx.x;
+    const int le = tid / NP, t = tid - le * NP;
+    const int lx = le % TX, ly = le / TX;
+    const int jx = tx0 + lx, jy = ty0 + ly;
+    const bool valid = lx < nvx && ly < nvy;
+    const int e = jy * nx + jx;
+    const float g = float(P.gamma);
+    const float* ctab = static_cast<const float*>(P.ctabR);
+    const float* rtab = static_cast<const float*>(P.rtabR);
+    const float Jx = ctab[(size_t)(valid ? jx : tx0) * CT + L::JAC];
+    const float Jy = rtab[(size_t)(valid ? jy : ty0) * CT + L::JAC];
+    const float dx = ctab[(size_t)(valid ? jx : tx0) * CT + L::DELTA];
+    const float dy = rtab[(size_t)(valid ? jy : ty0) * CT + L::DELTA];
+    const float iJx = rcp_(Jx), iJy = rcp_(Jy);
+
+    // neighbours of the tile (uniform over the block)
+    const int jxW = tx0 > 0 ? tx0 - 1 : (P.bcx == 0 ? nx - 1 : -1);
+    const int jxE = tx0 + nvx < nx ? tx0 + nvx : (P.bcx == 0 ? 0 : -1);
+    const int jyS = ty0 > 0 ? ty0 - 1 : (P.bcy == 0 ? ny - 1 : -1);
+    const int jyN = ty0 + nvy < ny ? ty0 + nvy : (P.bcy == 0 ? 0 : -1);
+
+    const float eps_e = (AV && valid) ? float(P.eps[e]) : 0.0f;
+
+    // ------------------------------------------------------------------ P0: loads + row sweep
+    double base[NP][kNeq];
+    float ur[NP][kNeq];
+    {
+        double uk[NP][kNeq];
+        if (valid) {
+            load_row<NP>(P.uin, nel, e, t, uk);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) uk[i][c] = 0.0;
+        }
+        if (valid && P.stage >= 2) {
+            double u0[NP][kNeq];
+            load_row<NP>(P.u0, nel, e, t, u0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) base[i][c] = rk_base_fma(P.stage, u0[i][c], uk[i][c]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) base[i][c] = uk[i][c];
+        }
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                ur[i][c] = float(uk[i][c]);
+                sU[le * ESTR + c * NPP + t * NP + i] = ur[i][c];
+            }
+    }
+    if constexpr (AV) {
+        if (t == 0) sEps[le] = eps_e;
+    }
+    {   // halo tasks: one line of a face-adjacent element outside the tile
+        int k = tid;
+        if (k < HX) {                                  // west halo: east trace of row j
+            const int r = k / NP, j = k - r * NP;
+            if (r < nvy && jxW >= 0) {
+                const int en = (ty0 + r) * nx + jxW;
+                float geps = 0.0f;
+                if constexpr (BR0) {
+                    const float epsn = float(P.eps[en]);
+                    geps = epsn * rcp_(ctab[(size_t)jxW * CT + L::JAC]);
+                    if (j == 0) sEpsH[r] = epsn;
+                }
+                halo_line<NP, BR0, true>(P.uin, nel, en, j, op.lR, op.dR, geps, &sHW[r * NP + j],
+                                         &sGW[BR0 ? r * NP + j : 0], HX);
+            }
+        } else if ((k -= HX) < HX) {                   // east halo: west trace of row j
+            const int r = k / NP, j = k - r * NP;
+            if (r < nvy && jxE >= 0) {
+                const int en = (ty0 + r) * nx + jxE;
+                float geps = 0.0f;
+                if constexpr (BR0) {
+                    const float epsn = float(P.eps[en]);
+                    geps = epsn * rcp_(ctab[(size_t)jxE * CT + L::JAC]);
+                    if (j == 0) sEpsH[TY + r] = epsn;
+                }
+                halo_line<NP, BR0, true>(P.uin, nel, en, j, op.lL, op.dL, geps, &sHE[r * NP + j],
+                                         &sGE[BR0 ? r * NP + j : 0], HX);
+            }
+        } else if ((k -= HX) < HY) {                   // south halo: north trace of column i
+            const int l = k / NP, i = k - l * NP;
+            if (l < nvx && jyS >= 0) {
+                const int en = jyS * nx + tx0 + l;
+                float geps = 0.0f;
+                if constexpr (BR0) {
+                    const float epsn = float(P.eps[en]);
+                    geps = epsn * rcp_(rtab[(size_t)jyS * CT + L::JAC]);
+                    if (i == 0) sEpsH[2 * TY + l] = epsn;
+                }
+                halo_line<NP, BR0, false>(P.uin, nel, en, i, op.lR, op.dR, geps, &sHS[l * NP + i],
+                                          &sGS[BR0 ? l * NP + i : 0], HY);
+            }
+        } else if ((k -= HY) < HY) {                   // north halo: south trace of column i
+            const int l = k / NP, i = k - l * NP;
+            if (l < nvx && jyN >= 0) {
+                const int en = jyN * nx + tx0 + l;
+                float geps = 0.0f;
+                if constexpr (BR0) {
+                    const float epsn = float(P.eps[en]);
+                    geps = epsn * rcp_(rtab[(size_t)jyN * CT + L::JAC]);
+                    if (i == 0) sEpsH[2 * TY + TX + l] = epsn;
+                }
+                halo_line<NP, BR0, false>(P.uin, nel, en, i, op.lL, op.dL, geps, &sHN[l * NP + i],
+                                          &sGN[BR0 ? l * NP + i : 0], HY);
+            }
+        }
+    }
+    // xi sweep of the own row, straight from the registers (on every lane when the matrix
+    // cores do the contractions: invalid lanes sweep the zero rows they stored)
+    float rowV[NP][kNeq];
+    float HW[kNeq], HE[kNeq], uE[kNeq], gE[kNeq];
+    LineMats<MF ? NP : 1> mats;
+    if constexpr (MF) mats.init(op);
+    if (MF || valid) {
+        float uW[kNeq], gW[kNeq];
+        if constexpr (MF)
+            sweep_mf<NP, CAPT, 0>(mats, ur, g, eps_e * iJx, rowV, HW, HE, uW, uE, gW, gE);
+        else
+            sweep<NP, CAPT, 0>(op, ur, g, eps_e * iJx, rowV, HW, HE, uW, uE, gW, gE);
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                sXW[c * FS + le * NP + t] = uW[c];
+                if constexpr (BR0) sXG[c * FS + le * NP + t] = gW[c];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P1: eta sweep
+    float HS[kNeq], HN[kNeq], uN[kNeq], gN[kNeq];
+    if (MF || valid) {
+        float* col = sU + le * ESTR + t;   // column i = t: [c][j] at c*NPP + j*NP
+        float uc[NP][kNeq];
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) uc[j][c] = col[c * NPP + j * NP];
+        float V[NP][kNeq], uS[kNeq], gS[kNeq];
+        if constexpr (MF)
+            sweep_mf<NP, CAPT, 1>(mats, uc, g, eps_e * iJy, V, HS, HN, uS, uN, gS, gN);
+        else
+            sweep<NP, CAPT, 1>(op, uc, g, eps_e * iJy, V, HS, HN, uS, uN, gS, gN);
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c) {
+                sYS[c * FS + le * NP + t] = uS[c];
+                if constexpr (BR0) sYG[c * FS + le * NP + t] = gS[c];
+            }
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) col[c * NPP + j * NP] = V[j][c];
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P2: faces (once each)
+    // The row's xi volume term joins the eta one in LDS: sU then holds -(D H)_xi/Jx - (D H)_eta/Jy
+    // per point, and rowV needs no registers through the face phase.
+    if (valid) {
+        float* row = sU + le * ESTR + t * NP;
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c)
+#pragma unroll
+            for (int i = 0; i < NP; ++i) row[c * NPP + i] = -iJy * row[c * NPP + i] - iJx * rowV[i][c];
+    }
+    float hE[kNeq], hN[kNeq];
+    if (valid) {
+        {   // E face of row j = t
+            const bool in_tile = lx + 1 < nvx;
+            if (in_tile || jxE >= 0) {
+                const float* su = in_tile ? &sXW[(le + 1) * NP + t] : &sHE[ly * NP + t];
+                const int cs = in_tile ? FS : HX;
+                float up[kNeq], gp[kNeq];
+                float tau = 0.0f;
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) up[c] = su[c * cs];
+                if constexpr (BR0) {
+                    const float* sg = in_tile ? &sXG[(le + 1) * NP + t] : &sGE[ly * NP + t];
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) gp[c] = sg[c * cs];
+                    const float epsn = in_tile ? sEps[le + 1] : sEpsH[TY + ly];
+                    const int jxn = in_tile ? jx + 1 : jxE;
+                    tau = br0_tau_f(eps_e, epsn, dx, ctab[(size_t)jxn * CT + L::DELTA], P.p);
+                }
+                face_h<FLUX, 0, BR0>(uE, up, gE, gp, tau, g, hE);
+            } else {
+                boundary_h<0, BR0>(uE, gE, g, hE);
+            }
+        }
+        {   // N face of column i = t
+            const bool in_tile = ly + 1 < nvy;
+            if (in_tile || jyN >= 0) {
+                const float* su = in_tile ? &sYS[(le + TX) * NP + t] : &sHN[lx * NP + t];
+                const int cs = in_tile ? FS : HY;
+                float up[kNeq], gp[kNeq];
+                float tau = 0.0f;
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) up[c] = su[c * cs];
+                if constexpr (BR0) {
+                    const float* sg = in_tile ? &sYG[(le + TX) * NP + t] : &sGN[lx * NP + t];
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) gp[c] = sg[c * cs];
+                    const float epsn = in_tile ? sEps[le + TX] : sEpsH[2 * TY + TX + lx];
+                    const int jyn = in_tile ? jy + 1 : jyN;
+                    tau = br0_tau_f(eps_e, epsn, dy, rtab[(size_t)jyn * CT + L::DELTA], P.p);
+                }
+                face_h<FLUX, 1, BR0>(uN, up, gN, gp, tau, g, hN);
+            } else {
+                boundary_h<1, BR0>(uN, gN, g, hN);
+            }
+        }
+    }
+    {   // tile-edge faces (W edge: TY*NP points, S edge: TX*NP points) on the first threads
+        int k = tid;
+        if (k < HX) {
+            const int r = k / NP, j = k - r * NP;
+            if (r < nvy) {
+                const int lo = r * TX;   // element at lx = 0 of row r
+                float uo[kNeq], go[kNeq], um[kNeq], gm[kNeq], fh[kNeq];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) {
+                    uo[c] = sXW[c * FS + lo * NP + j];
+                    if constexpr (BR0) go[c] = sXG[c * FS + lo * NP + j];
+                }
+                if (jxW >= 0) {
+                    float tau = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) {
+                        um[c] = sHW[c * HX + r * NP + j];
+                        if constexpr (BR0) gm[c] = sGW[c * HX + r * NP + j];
+                    }
+                    if constexpr (BR0)
+                        tau = br0_tau_f(sEpsH[r], sEps[lo], ctab[(size_t)jxW * CT + L::DELTA],
+                                        ctab[(size_t)tx0 * CT + L::DELTA], P.p);
+                    face_h<FLUX, 0, BR0>(um, uo, gm, go, tau, g, fh);
+                } else {
+                    boundary_h<0, BR0>(uo, go, g, fh);
+                }
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sF0x[c * HX + r * NP + j] = fh[c];
+            }
+        } else if ((k -= HX) < HY) {
+            const int l = k / NP, i = k - l * NP;
+            if (l < nvx) {
+                const int lo = l;        // element at ly = 0 of column l
+                float uo[kNeq], go[kNeq], um[kNeq], gm[kNeq], fh[kNeq];
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) {
+                    uo[c] = sYS[c * FS + lo * NP + i];
+                    if constexpr (BR0) go[c] = sYG[c * FS + lo * NP + i];
+                }
+                if (jyS >= 0) {
+                    float tau = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) {
+                        um[c] = sHS[c * HY + l * NP + i];
+                        if constexpr (BR0) gm[c] = sGS[c * HY + l * NP + i];
+                    }
+                    if constexpr (BR0)
+                        tau = br0_tau_f(sEpsH[2 * TY + l], sEps[lo], rtab[(size_t)jyS * CT + L::DELTA],
+                                        rtab[(size_t)ty0 * CT + L::DELTA], P.p);
+                    face_h<FLUX, 1, BR0>(um, uo, gm, go, tau, g, fh);
+                } else {
+                    boundary_h<1, BR0>(uo, go, g, fh);
+                }
+#pragma unroll
+                for (int c = 0; c < kNeq; ++c) sF0y[c * HY + l * NP + i] = fh[c];
+            }
+        }
+    }
+    __syncthreads();   // every trace read is done: the trace slots now take the face fluxes
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            sXW[c * FS + le * NP + t] = hE[c];
+            sYS[c * FS + le * NP + t] = hN[c];
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ P3: corrections, combine
+    if (valid) {   // column i = t: eta corrections added to the eta volume term in place
+        const float* hs = ly > 0 ? &sYS[(le - TX) * NP + t] : &sF0y[lx * NP + t];
+        const int cs = ly > 0 ? FS : HY;
+        float dS[kNeq], dN[kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            dS[c] = -iJy * (hs[c * cs] - HS[c]);
+            dN[c] = -iJy * (hN[c] - HN[c]);
+        }
+        float* col = sU + le * ESTR + t;
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int c = 0; c < kNeq; ++c)
+                col[c * NPP + j * NP] += dS[c] * op.gL[j] + dN[c] * op.gR[j];
+    }
+    __syncthreads();
+
+    bool bad = false, bad_u = false;
+    float lam_row = 0.0f;
+    float rho_new[NP];
+    if (valid) {   // row j = t
+        const float* hw = lx > 0 ? &sXW[(le - 1) * NP + t] : &sF0x[ly * NP + t];
+        const int cs = lx > 0 ? FS : HX;
+        float dW[kNeq], dE[kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            dW[c] = hw[c * cs] - HW[c];
+            dE[c] = hE[c] - HE[c];
+        }
+        const float* row = sU + le * ESTR + t * NP;
+        const int stage = P.stage;
+        const double dt = stage > 0 ? P.ctl->dt : 0.0;
+        const double cdt = stage == 1 ? dt : (stage == 2 ? 0.25 * dt : (2.0 / 3.0) * dt);
+        float unf[NP][kNeq];
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            double* po = P.uout + ((size_t)c * nel + e) * NPP + t * NP;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const float d = row[c * NPP + i] - iJx * (dW[c] * op.gL[i] + dE[c] * op.gR[i]);
+                double un;
+                if (stage == 0) {
+                    un = double(d);
+                } else {
+                    bad |= !isfinite(d);
+                    un = rk_final_fma(base[i][c], double(d), cdt);
+                    if (stage == 3) bad_u |= !isfinite(un);
+                }
+                po[i] = un;
+                unf[i][c] = float(un);
+            }
+        }
+        if (stage > 0 && (P.want_lambda || AV)) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                rho_new[i] = unf[i][0];
+                lam_row = nan_max(lam_row, point_wavespeed<float>(unf[i], g));
+            }
+        }
+    }
+    if (P.stage > 0) {
+        const int any_bad = __syncthreads_or(bad ? 1 : 0);
+        const int any_bad_u = (P.stage == 3) ? __syncthreads_or(bad_u ? 1 : 0) : 0;
+        if (tid == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
+    }
+
+    // ------------------------------------------------------------------ P4: epilogue
+    if (P.stage == 0 || !(P.want_lambda || AV)) return;
+    __syncthreads();   // the row reads of sU are done: its element slots become scratch
+    float* sT = sU + le * ESTR;   // [a*NP + j] row-transformed density, [NPP + k] lambda,
+                                  // [NPP + NP + k] total and [NPP + 2 NP + k] high modal energy
+    if (valid) {
+        if constexpr (AV) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                float s = 0.0f;
+#pragma unroll
+                for (int i = 0; i < NP; ++i) s += op.Vinv[a * NP + i] * rho_new[i];
+                sT[a * NP + t] = s;
+            }
+        }
+        sT[NPP + t] = lam_row;
+    }
+    __syncthreads();
+    if constexpr (AV) {
+        if (valid) {   // modes (a, b = t): energies of column b of the modal coefficients
+            const int b = t;
+            float et = 0.0f, eh = 0.0f, ecol = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                float s = 0.0f;
+#pragma unroll
+                for (int jj = 0; jj < NP; ++jj) s += sT[a * NP + jj] * op.Vinv[b * NP + jj];
+                et += s * s;
+                if (a == NP - 1)
+                    eh += s * s;
+                else
+                    ecol += s * s;
+            }
+            if (b == NP - 1) eh += ecol;
+            sT[NPP + NP + t] = et;
+            sT[NPP + 2 * NP + t] = eh;
+        }
+        __syncthreads();
+    }
+    float lm_el = 0.0f;
+    if (valid && t == 0) {
+        float lm = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) lm = nan_max(lm, sT[NPP + k]);
+        lm = nan_max(float(kEps64), lm);
+        if constexpr (AV) {
+            float etot = 0.0f, ehigh = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                etot += sT[NPP + NP + k];
+                ehigh += sT[NPP + 2 * NP + k];
+            }
+            etot += float(P.eps_floor);
+            const float sigma = persson_sigma_hw(ehigh, etot, float(P.s0), float(P.kappa), float(P.eps_floor));
+            const float h = dx < dy ? dx : dy;
+            const int pp = P.p > 1 ? P.p : 1;
+            P.eps_out[e] = sigma > 0.0f ? double(float(P.c_av) * sigma * (h / float(pp)) * lm) : 0.0;
+        }
+        lm_el = lm;
+    }
+    if (P.want_lambda) {   // block maximum of the element maxima -> one atomic per block
+        float m = lm_el;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+        if ((tid & 63) == 0) sWave[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            float mm = 0.0f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) mm = nan_max(mm, sWave[w]);
+            atomicMax(&P.ctl->lam_bits, (unsigned long long)__double_as_longlong(double(mm)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+template <int NP>
+inline OpsM<NP> make_ops_mixed(const HostOps& h) {
+    OpsM<NP> o;
+    for (int a = 0; a < NP * NP; ++a) {
+        o.D[a] = float(h.D[a]);
+        o.Vinv[a] = float(h.Vinv[a]);
+    }
+    for (int k = 0; k < NP; ++k) {
+        double sl = 0.0, sr = 0.0;   // dL[k] = sum_m lL[m] D[m,k]
+        for (int m = 0; m < NP; ++m) {
+            sl += h.lL[m] * h.D[m * NP + k];
+            sr += h.lR[m] * h.D[m * NP + k];
+        }
+        o.dL[k] = float(sl);
+        o.dR[k] = float(sr);
+        o.lL[k] = float(h.lL[k]);
+        o.lR[k] = float(h.lR[k]);
+        o.gL[k] = float(h.gLxi[k]);
+        o.gR[k] = float(h.gRxi[k]);
+    }
+    return o;
+}
+
+// Tile shapes per degree (elements per tile row x rows); block = TX*TY*Np threads.
+#ifndef FRF_M4X
+#define FRF_M4X 8
+#endif
+#ifndef FRF_M4Y
+#define FRF_M4Y 8
+#endif
+template <int NP> struct TileM { static constexpr int TX = 16, TY = 4; };
+template <> struct TileM<4> { static constexpr int TX = FRF_M4X, TY = FRF_M4Y; };
+template <> struct TileM<2> { static constexpr int TX = 16, TY = 8; };
+template <> struct TileM<6> { static constexpr int TX = 8, TY = 4; };
+
+}  // namespace frf

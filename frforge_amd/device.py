@@ -25,7 +25,9 @@ STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
-            "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy")
+            "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy",
+            "frf_batch_create", "frf_batch_tune", "frf_batch_run", "frf_batch_tile",
+            "frf_batch_last_error", "frf_batch_destroy")
 
 
 class FrfProblem(C.Structure):
@@ -88,6 +90,16 @@ def load_library(path: str = LIB_PATH):
     L.frf_last_error.restype = C.c_char_p
     L.frf_last_error.argtypes = [vp]
     L.frf_destroy.argtypes = [vp]
+    L.frf_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.frf_batch_tune.argtypes = [vp, C.c_int32]
+    L.frf_batch_run.argtypes = [vp, dp, dp, dp, C.POINTER(C.c_int64), C.POINTER(FrfRunResult), dp]
+    L.frf_batch_tile.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.frf_batch_last_error.restype = C.c_char_p
+    L.frf_batch_last_error.argtypes = [vp]
+    L.frf_batch_destroy.argtypes = [vp]
+    for name in ("frf_batch_create", "frf_batch_tune", "frf_batch_run", "frf_batch_tile",
+                 "frf_batch_destroy"):
+        getattr(L, name).restype = C.c_int32
     for name in ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
                  "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt",
                  "frf_rk3_step", "frf_run", "frf_advance", "frf_profile_steps", "frf_destroy"):
@@ -220,3 +232,57 @@ class DeviceContext:
 
     def stream(self) -> int:
         return int(self.lib.frf_stream(self.h) or 0)
+
+
+class DeviceBatch:
+    """frf_batch: independent contexts (FP32-mixed form) advanced together."""
+
+    def __init__(self, contexts):
+        self.lib = load_library()
+        self.contexts = list(contexts)   # kept alive: the batch holds their pointers
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = self.lib.frf_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc != FRF_OK:
+            raise FrfError(f"frf_batch_create: {ERRORS.get(rc, rc)}: {self.lib.frf_last_error(None).decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != FRF_OK:
+            raise FrfError(f"{what}: {ERRORS.get(rc, rc)}: {self.lib.frf_batch_last_error(self.h).decode()}")
+
+    def tune(self, repeats: int = 3) -> None:
+        self._chk(self.lib.frf_batch_tune(self.h, int(repeats)), "frf_batch_tune")
+
+    def tile(self, i: int):
+        tx, ty = C.c_int32(), C.c_int32()
+        self._chk(self.lib.frf_batch_tile(self.h, int(i), C.byref(tx), C.byref(ty)), "frf_batch_tile")
+        return tx.value, ty.value
+
+    def run(self, t_final, cfl, fixed_dt, max_steps):
+        n = len(self.contexts)
+        tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_final, dtype=np.float64), (n,)))
+        cf = np.ascontiguousarray(np.broadcast_to(np.asarray(cfl, dtype=np.float64), (n,)))
+        fd = np.ascontiguousarray(np.broadcast_to(np.asarray(fixed_dt, dtype=np.float64), (n,)))
+        ms = np.ascontiguousarray(np.broadcast_to(np.asarray(max_steps, dtype=np.int64), (n,)))
+        res = (FrfRunResult * n)()
+        el = C.c_double()
+        self._chk(self.lib.frf_batch_run(self.h, _dp(tf), _dp(cf), _dp(fd),
+                                         ms.ctypes.data_as(C.POINTER(C.c_int64)), res, C.byref(el)),
+                  "frf_batch_run")
+        for c in self.contexts:
+            c.host_dirty = True
+        out = [{"status": STATUS[r.status], "n_steps": r.n_steps, "t": r.t, "fail_stage": r.fail_stage}
+               for r in res]
+        return out, el.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

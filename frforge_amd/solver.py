@@ -20,7 +20,7 @@ from typing import Optional
 import numpy as np
 
 from .capturing import AbstractCapturingMethod, NullCapturing
-from .device import DeviceContext
+from .device import DeviceBatch, DeviceContext
 from .equations import Euler2D
 from .mesh import PeriodicBC, TransmissiveBC
 from .state import SolutionState2D
@@ -154,3 +154,43 @@ def integrate(state: SolutionState2D, eq: Euler2D, method: Optional[AbstractCapt
     if tsym == "ssp_rk2":
         raise NotImplementedError("SSP-RK2 is not implemented on the device path")
     raise ValueError(f"Unknown time scheme {tsym} (use ssp_rk3 or ssp_rk2)")
+
+
+def ssp_rk3_batch(states, eq: Euler2D, methods, t_final, cfl=0.2, dt=None, max_steps=10**7,
+                  tune: bool = True):
+    """Integrate independent states together, as a sweep of ``ssp_rk3`` calls would, in one
+    device loop (frf_batch_run): instances sharing degree, flux and capturing method advance with
+    one stage launch per group, replayed from a captured HIP graph.  FP32-mixed states only.
+
+    ``methods``, ``t_final``, ``cfl``, ``dt`` and ``max_steps`` are one per state or a single value
+    for all; ``dt=None`` (or a non-positive entry) selects the CFL step.  ``tune`` runs the tile
+    autotuner first.  Returns ([{"status", "n_steps", "t"} per state], device milliseconds of the
+    stepping loop)."""
+    states = list(states)
+    n = len(states)
+    if n == 0:
+        return [], 0.0
+    methods = list(methods) if isinstance(methods, (list, tuple)) else [methods] * n
+    if len(methods) != n:
+        raise ValueError("one capturing method per state")
+    if dt is None:
+        fixed = np.full(n, -1.0)
+    else:
+        fixed = np.array([(-1.0 if d is None else float(d)) for d in np.broadcast_to(np.asarray(dt, dtype=object), (n,))])
+    devs = []
+    for st, m in zip(states, methods):
+        if st.precision != "fp32_mixed":
+            raise NotImplementedError("the batched form runs FP32-mixed states only")
+        dev = _device(st, eq, m)
+        _push(st, dev)
+        devs.append(dev)
+    batch = DeviceBatch(devs)
+    try:
+        if tune:
+            batch.tune()
+        res, ms = batch.run(t_final, cfl, fixed, max_steps)
+    finally:
+        batch.close()
+    for st, dev in zip(states, devs):
+        _pull(st, dev)
+    return [{"status": r["status"], "n_steps": r["n_steps"], "t": r["t"]} for r in res], ms

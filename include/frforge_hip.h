@@ -136,6 +136,31 @@ const char* frf_last_error(const frf_ctx* ctx);
 
 int32_t frf_destroy(frf_ctx* ctx);
 
+/* ---- Batched form: many independent instances advanced together in one loop.
+ * No single reference counterpart: it runs what the reference does as a sweep of independent
+ * ssp_rk3! / integrate! calls over meshes, degrees and fluxes (the case runners' loops,
+ * Cases2D_Riemann.jl:194-255, and the invent/scoring sweeps) as one job.  Instances are contexts
+ * set up as above in the FP32-mixed form.  Instances sharing degree, flux and capturing method
+ * advance with one stage launch per group; the step sequence of all groups is captured once in
+ * a HIP graph and replayed.  A batch keeps pointers to its contexts: destroy it first. */
+typedef struct frf_batch frf_batch;
+int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch** out);
+
+/* Autotuner: times each group's candidate tile shapes on its instances' current states (stage-1
+ * launches into the scratch stage buffers; states and times are left unchanged) and keeps the
+ * fastest.  Optional: untuned groups use the default shape of their degree. */
+int32_t frf_batch_tune(frf_batch* b, int32_t repeats);
+
+/* As frf_run for every instance i with its own t_final[i], cfl[i], fixed_dt[i], max_steps[i];
+ * results[i] as frf_run's.  elapsed_ms (may be NULL): device time of the stepping loop. */
+int32_t frf_batch_run(frf_batch* b, const double* t_final, const double* cfl, const double* fixed_dt,
+                      const int64_t* max_steps, frf_run_result* results, double* elapsed_ms);
+
+/* Tile shape (elements per workgroup in x and y) used for instance i. */
+int32_t frf_batch_tile(const frf_batch* b, int32_t i, int32_t* tx, int32_t* ty);
+const char* frf_batch_last_error(const frf_batch* b);
+int32_t frf_batch_destroy(frf_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

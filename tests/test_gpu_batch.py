@@ -1,0 +1,118 @@
+"""The batched form (frf_batch_*): independent FP32-mixed instances of uneven size, degree, flux,
+capturing and boundary kind advanced in one graph-replayed loop must give exactly what the
+single-instance path gives for each of them, and stay within FP32 resolution of the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from frforge_amd import compute_dt, ssp_rk3, ssp_rk3_batch
+from frforge_amd.device import DeviceBatch, FrfError
+from frforge_amd.solver import _device, _push
+
+from helpers import EQ, make_case, rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+# (p, nx, ny, bc, flux, capturing, ic)
+CASES = [
+    (1, 13, 9, "periodic", "rusanov", "null", "vortex"),
+    (2, 37, 21, "transmissive", "hllc", "persson_br0", "riemann"),
+    (3, 20, 20, "transmissive", "rusanov", "persson_br0", "riemann"),
+    (3, 9, 30, "periodic", "hllc", "null", "vortex"),
+    (4, 17, 11, "transmissive", "rusanov", "persson_dd", "riemann"),
+    (5, 10, 12, "transmissive", "hllc", "persson_br0", "riemann"),
+    (3, 33, 7, "transmissive", "rusanov", "persson_br0", "riemann"),
+]
+
+
+def _make(i, precision="fp32_mixed"):
+    p, nx, ny, bc, flux, capt, ic = CASES[i]
+    return make_case(p, nx, ny, bc=bc, flux=flux, capturing=capt, ic=ic, seed=40 + i,
+                     precision=precision)
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_fixed_dt_matches_single_runs(tune):
+    built = [_make(i) for i in range(len(CASES))]
+    states = [b[0] for b in built]
+    methods = [b[1] for b in built]
+    dts = [0.5 * compute_dt(s, EQ, cfl=0.1) for s in states]
+    singles = [_make(i) for i in range(len(CASES))]
+    ref = []
+    for (st, m, _), dt in zip(singles, dts):
+        ref.append(ssp_rk3(st, EQ, m, math.inf, dt=dt, max_steps=5))
+    res, ms = ssp_rk3_batch(states, EQ, methods, math.inf, dt=dts, max_steps=5, tune=tune)
+    print(f"batch of {len(states)} (tune={tune}): {ms:.3f} ms", flush=True)
+    for i, (r, rr) in enumerate(zip(res, ref)):
+        assert (r["status"], r["n_steps"], r["t"]) == (rr["status"], rr["n_steps"], rr["t"]) == ("max_steps", 5, rr["t"])
+        if not tune:   # same tile shapes as the single-instance launches: identical bits
+            assert np.array_equal(states[i].u, singles[i][0].u), f"instance {i} differs from its single run"
+        else:          # another tile shape may round a few halo traces differently (FP32 ulps)
+            err = rel_err_per_component(states[i].u, singles[i][0].u)
+            assert err <= 1e-6, f"instance {i}: {err:.3e}"
+
+
+def test_batch_cfl_runs_to_own_t_final():
+    built = [_make(i) for i in range(len(CASES))]
+    states = [b[0] for b in built]
+    methods = [b[1] for b in built]
+    t_final = [0.002 * (1 + i % 3) for i in range(len(CASES))]
+    singles = [_make(i) for i in range(len(CASES))]
+    oracle_status = [orc.run(st.u, 0.0, tf, cfl=0.1)[0] for (st, _, orc), tf in zip(built, t_final)]
+    ref = [ssp_rk3(st, EQ, m, tf, cfl=0.1) for (st, m, _), tf in zip(singles, t_final)]
+    res, _ = ssp_rk3_batch(states, EQ, methods, t_final, cfl=0.1)
+    for i, (r, rr) in enumerate(zip(res, ref)):
+        print(f"instance {i}: {r} (fp64 oracle: {oracle_status[i]})", flush=True)
+        assert r == rr, f"instance {i}: {r} != {rr}"
+        assert r["status"] == oracle_status[i]   # a run that diverges does so in FP64 as well
+        if r["status"] == "ok":
+            assert r["t"] == t_final[i]
+            assert rel_err_per_component(states[i].u, singles[i][0].u) <= 1e-6
+    assert sum(r["status"] == "ok" for r in res) >= len(CASES) - 1
+
+
+def test_batch_against_fp64_oracle():
+    """Per-instance error of the batched FP32-mixed form against the FP64 oracle."""
+    idx = [1, 2, 5]
+    built = [_make(i) for i in idx]
+    states = [b[0] for b in built]
+    t_final = 0.003
+    oracle_out = [orc.run(st.u, 0.0, t_final, cfl=0.1) for st, _, orc in built]
+    res, _ = ssp_rk3_batch(states, EQ, [b[1] for b in built], t_final, cfl=0.1)
+    for k, (st, r, (st_o, n_o, t_o, u_o)) in enumerate(zip(states, res, oracle_out)):
+        err = rel_err_per_component(st.u, u_o)
+        print(f"instance {idx[k]}: n={r['n_steps']} (fp64 {n_o}) rel err vs fp64 oracle {err:.3e}", flush=True)
+        assert r["status"] == st_o == "ok" and abs(r["n_steps"] - n_o) <= 1
+        assert err <= 1e-5
+
+
+def test_batch_tune_leaves_states_and_picks_a_candidate():
+    built = [_make(i) for i in range(len(CASES))]
+    devs = []
+    for st, m, _ in built:
+        dev = _device(st, EQ, m)
+        _push(st, dev)
+        devs.append(dev)
+    before = [st.u.copy(order="F") for st, _, _ in built]
+    b = DeviceBatch(devs)
+    try:
+        b.tune(2)
+        tiles = [b.tile(i) for i in range(len(devs))]
+    finally:
+        b.close()
+    for st, dev, u0 in zip([x[0] for x in built], devs, before):
+        out = np.empty_like(u0, order="F")
+        t = dev.download(out)
+        assert t == st.t and np.array_equal(out, u0)
+    assert all(tx * ty >= 32 for tx, ty in tiles)
+
+
+def test_batch_rejects_fp64_contexts():
+    st, m, _ = _make(0, precision="fp64")
+    dev = _device(st, EQ, m)
+    _push(st, dev)
+    with pytest.raises(FrfError, match="FP32-mixed"):
+        DeviceBatch([dev])
+    with pytest.raises(NotImplementedError):
+        ssp_rk3_batch([st], EQ, [m], 0.01)

@@ -1,0 +1,94 @@
+"""Shock-quality metrics of the reference's verification suite (Metrics.jl:75-255), used by the
+Sod case runner: overshoot, smooth-region mask, excess dissipation and 10-90 % shock thickness.
+Inputs are solution-point samples sorted by x."""
+from __future__ import annotations
+
+import numpy as np
+
+EPS = np.finfo(np.float64).eps
+
+
+def overshoot_metric(u_min, u_max, u_ref_min, u_ref_max):
+    """(eta_over, eta_under, eta) relative to reference bounds (Metrics.jl:81-87)."""
+    d = max(abs(u_ref_max - u_ref_min), EPS)
+    over = max(0.0, float(u_max) - float(u_ref_max)) / d
+    under = max(0.0, float(u_ref_min) - float(u_min)) / d
+    return over, under, max(over, under)
+
+
+def mean_sp_spacing(x) -> float:
+    """(x_last - x_first) / (n - 1) (Metrics.jl:124-128)."""
+    n = len(x)
+    return 1.0 if n < 2 else (x[-1] - x[0]) / (n - 1)
+
+
+def _grad(x, rho, i):
+    return abs(rho[i] - rho[i - 1]) / max(abs(x[i] - x[i - 1]), 1e-30)
+
+
+def shock_thickness_sp(x, rho, window_sp: int = 30) -> float:
+    """10 %-90 % rise distance of the steepest density jump in mean solution-point spacings,
+    plateaus from the medians of the outer thirds of a +-window_sp window (Metrics.jl:138-206).
+    Indices below are 0-based; the window, thirds and medians follow the 1-based originals."""
+    n = len(x)
+    if n < 4:
+        return float("nan")
+    imax, gmax = 1, 0.0
+    for i in range(1, n):
+        gi = _grad(x, rho, i)
+        if gi > gmax:
+            gmax, imax = gi, i
+    i0, i1 = max(0, imax - window_sp), min(n - 1, imax + window_sp)
+    w = i1 - i0 + 1
+    nl = max(1, w // 3)
+    left = sorted(rho[i0:i0 + nl])
+    right = sorted(rho[i1 - nl + 1:i1 + 1])
+    rl = left[max(1, len(left) // 2) - 1]
+    rr = right[max(1, len(right) // 2) - 1]
+    rmin, rmax = min(rl, rr), max(rl, rr)
+    d = rmax - rmin
+    if abs(d) < 1e-14:
+        return float("nan")
+    lo, hi = rmin + 0.1 * d, rmin + 0.9 * d
+    i10 = i90 = imax
+    if rr >= rl:   # rising
+        while i10 > i0 and rho[i10] > lo:
+            i10 -= 1
+        while i90 < i1 and rho[i90] < hi:
+            i90 += 1
+    else:
+        while i10 < i1 and rho[i10] > lo:
+            i10 += 1
+        while i90 > i0 and rho[i90] < hi:
+            i90 -= 1
+    return abs(x[i90] - x[i10]) / mean_sp_spacing(x)
+
+
+def excess_dissipation(u_method, u_ref, mask=None) -> float:
+    """sum_S |u_m - u_ref| / (sum_S |u_ref| + 1e-30) over the mask (Metrics.jl:208-223)."""
+    num = den = 0.0
+    for i in range(len(u_method)):
+        if mask is not None and not mask[i]:
+            continue
+        num += abs(u_method[i] - u_ref[i])
+        den += abs(u_ref[i])
+    return num / (den + 1e-30)
+
+
+def smooth_region_mask(x, rho, grad_frac: float = 0.1, exclude_windows=()):
+    """Points whose backward gradient is at most grad_frac of the largest, outside the windows
+    (Metrics.jl:228-255)."""
+    n = len(x)
+    g = np.zeros(n)
+    for i in range(1, n):
+        g[i] = _grad(x, rho, i)
+    g[0] = g[1]
+    thr = grad_frac * float(np.max(g))
+    mask = np.ones(n, dtype=bool)
+    for i in range(n):
+        if g[i] > thr:
+            mask[i] = False
+        for a, b in exclude_windows:
+            if a <= x[i] <= b:
+                mask[i] = False
+    return mask

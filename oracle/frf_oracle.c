@@ -1005,3 +1005,344 @@ int orc_run(orc_ctx* h, double* u, double* t, double t_final, double cfl, double
     *n_steps = n;
     return ORC_OK;
 }
+
+/* ================================================================ 1D Euler (config: Sod tube)
+ *
+ * Euler1D (Euler.jl:19-128), residual! 1D (Residual.jl:24-214), Persson sensor and BR0 / local DD
+ * 1D (PerssonAV.jl:103-352, SharedAV.jl:9-42), compute_dt / ssp_rk3_step! / ssp_rk3! 1D
+ * (SSP_RK3.jl:8-118).  Layout u[j,e,c] -> u[j + Np*(e + Nel*c)], Neq = 3.
+ * Pinned by the reference's snapshot fixture case sod_p2_ne64_persson_av (n_steps, density
+ * extrema, L1 vs the exact solution) and the closed forms in tests/test_oracle1d.py. */
+
+#define NEQ1 3
+
+struct orc1_ctx {
+    orc1_problem pb;
+    int Np, Nel;
+    double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
+    double *xv, *dx, *J;
+    double* LU; int* piv;
+    /* workspace */
+    double *uL, *uR, *fL, *fR, *sig, *eps, *g, *gL, *gR, *ghL, *ghR;
+};
+
+/* pressure(eq::Euler1D, U) (Euler.jl:22-27) */
+static inline double p1(double g, double rho, double ru, double E) {
+    double rs = jmax(rho, EPS);
+    double u = ru / rs;
+    return (g - 1.0) * (E - 0.5 * rs * u * u);
+}
+/* sound_speed (Euler.jl:40-44) and velocity (Euler.jl:35-37) */
+static inline double c1(double g, double rho, double ru, double E) {
+    double r = jmax(rho, EPS);
+    double p = jmax(p1(g, rho, ru, E), EPS);
+    return sqrt(g * p / r);
+}
+static inline double v1(double rho, double ru) { return ru / jmax(rho, EPS); }
+/* physical_flux (Euler.jl:75-81): u = rho u / rho without a floor */
+static inline void pflux1(double g, const double* U, double* F) {
+    double u = U[1] / U[0];
+    double p = p1(g, U[0], U[1], U[2]);
+    F[0] = U[1];
+    F[1] = U[1] * u + p;
+    F[2] = (U[2] + p) * u;
+}
+/* rusanov_flux (Rusanov.jl:10-25) with max_wave_speed (Euler.jl:103-109) */
+static void rusanov1(double g, const double* uL, const double* uR, double* out) {
+    double fL[3], fR[3];
+    pflux1(g, uL, fL);
+    pflux1(g, uR, fR);
+    double lam = jmax(fabs(v1(uL[0], uL[1])) + c1(g, uL[0], uL[1], uL[2]),
+                      fabs(v1(uR[0], uR[1])) + c1(g, uR[0], uR[1], uR[2]));
+    for (int c = 0; c < 3; ++c) out[c] = 0.5 * (fL[c] + fR[c]) - 0.5 * lam * (uR[c] - uL[c]);
+}
+
+orc1_ctx* orc1_create(const orc1_problem* pb, const double* xi, const double* w, const double* D,
+                      const double* lL, const double* lR, const double* gLxi, const double* gRxi,
+                      const double* V, const double* xv) {
+    if (pb->p < 0 || pb->nel < 1) return NULL;
+    if ((pb->bc_left == ORC_BC_PERIODIC) != (pb->bc_right == ORC_BC_PERIODIC)) return NULL;
+    orc1_ctx* h = (orc1_ctx*)calloc(1, sizeof(orc1_ctx));
+    h->pb = *pb;
+    int Np = pb->p + 1, Nel = pb->nel;
+    h->Np = Np; h->Nel = Nel;
+    h->xi = dup_arr(xi, Np); h->w = dup_arr(w, Np); h->D = dup_arr(D, Np * Np);
+    h->lL = dup_arr(lL, Np); h->lR = dup_arr(lR, Np);
+    h->gLxi = dup_arr(gLxi, Np); h->gRxi = dup_arr(gRxi, Np); h->V = dup_arr(V, Np * Np);
+    h->xv = dup_arr(xv, Nel + 1);
+    h->dx = (double*)malloc(sizeof(double) * Nel);
+    h->J = (double*)malloc(sizeof(double) * Nel);
+    for (int e = 0; e < Nel; ++e) {   /* Mesh1D.jl:44-47: dx = diff(xv), J = dx / 2 */
+        h->dx[e] = xv[e + 1] - xv[e];
+        h->J[e] = h->dx[e] / 2.0;
+    }
+    h->LU = (double*)malloc(sizeof(double) * Np * Np);
+    h->piv = (int*)malloc(sizeof(int) * Np);
+    lu_factor(h->V, Np, h->LU, h->piv);
+    size_t ne = (size_t)Nel * NEQ1;
+    h->uL = calloc(ne, 8); h->uR = calloc(ne, 8); h->fL = calloc(ne, 8); h->fR = calloc(ne, 8);
+    h->sig = calloc(Nel, 8); h->eps = calloc(Nel, 8);
+    h->g = calloc((size_t)Np * ne, 8);
+    h->gL = calloc(ne, 8); h->gR = calloc(ne, 8); h->ghL = calloc(ne, 8); h->ghR = calloc(ne, 8);
+    return h;
+}
+
+void orc1_destroy(orc1_ctx* h) {
+    if (!h) return;
+    void* a[] = {h->xi, h->w, h->D, h->lL, h->lR, h->gLxi, h->gRxi, h->V, h->xv, h->dx, h->J,
+                 h->LU, h->piv, h->uL, h->uR, h->fL, h->fR, h->sig, h->eps, h->g, h->gL, h->gR,
+                 h->ghL, h->ghR};
+    for (size_t k = 0; k < sizeof a / sizeof a[0]; ++k) free(a[k]);
+    free(h);
+}
+
+#define U1(u, j, e, c) (u)[(j) + Np * ((e) + Nel * (c))]
+#define E1(a, e, c) (a)[(e) + Nel * (c)]
+
+/* sense! 1D (PerssonAV.jl:116-135): u^ = V \ rho_e, e_high = u^[Np]^2, e_tot = sum u^2 + floor */
+void orc1_sense(orc1_ctx* h, const double* u, double* sigma) {
+    const int Np = h->Np, Nel = h->Nel;
+    const double s0 = h->pb.s0_factor * log10((double)(h->pb.p > 1 ? h->pb.p : 1));
+    double b[16];
+    for (int e = 0; e < Nel; ++e) {
+        for (int j = 0; j < Np; ++j) b[j] = U1(u, j, e, 0);
+        lu_solve(h->LU, h->piv, Np, b);
+        double eh = b[Np - 1] * b[Np - 1];
+        double et = 0.0;
+        for (int j = 0; j < Np; ++j) et += b[j] * b[j];
+        et += h->pb.eps_floor;
+        double se = log10(eh / et + h->pb.eps_floor);
+        sigma[e] = 1.0 / (1.0 + exp(-h->pb.kappa * (se - s0)));
+    }
+}
+
+/* element_viscosities 1D (PerssonAV.jl:155-176) */
+void orc1_element_viscosities(orc1_ctx* h, const double* u, const double* sigma, double* eps) {
+    const int Np = h->Np, Nel = h->Nel;
+    const double g = h->pb.gamma;
+    const int p = h->pb.p > 1 ? h->pb.p : 1;
+    for (int e = 0; e < Nel; ++e) {
+        eps[e] = 0.0;
+        if (sigma[e] <= 0.0) continue;
+        double m = 0.0;
+        for (int j = 0; j < Np; ++j) {
+            double r = U1(u, j, e, 0), ru = U1(u, j, e, 1), E = U1(u, j, e, 2);
+            m = jmax(m, fabs(v1(r, ru)) + c1(g, r, ru, E));
+        }
+        m = jmax(m, EPS);
+        eps[e] = h->pb.c_av * sigma[e] * (h->dx[e] / (double)p) * m;
+    }
+}
+
+/* residual! 1D (Residual.jl:161-214) with the capturing hooks of PerssonAVMethod */
+void orc1_residual(orc1_ctx* h, const double* u, double t, double* du) {
+    (void)t;
+    const int Np = h->Np, Nel = h->Nel;
+    const double g = h->pb.gamma;
+    /* traces (Interfaces.jl:49-72) */
+    for (int e = 0; e < Nel; ++e)
+        for (int c = 0; c < NEQ1; ++c) {
+            double sL = 0.0, sR = 0.0;
+            for (int j = 0; j < Np; ++j) {
+                sL += h->lL[j] * U1(u, j, e, c);
+                sR += h->lR[j] * U1(u, j, e, c);
+            }
+            E1(h->uL, e, c) = sL;
+            E1(h->uR, e, c) = sR;
+        }
+    /* interface fluxes (Residual.jl:24-115) */
+    double um[3], up[3], f[3];
+    for (int e = 0; e < Nel - 1; ++e) {
+        for (int c = 0; c < 3; ++c) { um[c] = E1(h->uR, e, c); up[c] = E1(h->uL, e + 1, c); }
+        rusanov1(g, um, up, f);
+        for (int c = 0; c < 3; ++c) { E1(h->fR, e, c) = f[c]; E1(h->fL, e + 1, c) = f[c]; }
+    }
+    const int periodic = h->pb.bc_left == ORC_BC_PERIODIC;
+    for (int c = 0; c < 3; ++c) {
+        up[c] = E1(h->uL, 0, c);
+        um[c] = periodic ? E1(h->uR, Nel - 1, c) : E1(h->uL, 0, c);
+    }
+    rusanov1(g, um, up, f);
+    for (int c = 0; c < 3; ++c) {
+        E1(h->fL, 0, c) = f[c];
+        if (periodic) E1(h->fR, Nel - 1, c) = f[c];
+    }
+    if (!periodic) {
+        for (int c = 0; c < 3; ++c) { um[c] = E1(h->uR, Nel - 1, c); up[c] = E1(h->uR, Nel - 1, c); }
+        rusanov1(g, um, up, f);
+        for (int c = 0; c < 3; ++c) E1(h->fR, Nel - 1, c) = f[c];
+    }
+    /* volume + corrections */
+    double F[16][3], Uj[3], fLc[3], fRc[3];
+    for (int e = 0; e < Nel; ++e) {
+        for (int j = 0; j < Np; ++j) {
+            for (int c = 0; c < 3; ++c) Uj[c] = U1(u, j, e, c);
+            pflux1(g, Uj, F[j]);
+        }
+        for (int c = 0; c < 3; ++c) {
+            double sL = 0.0, sR = 0.0;
+            for (int j = 0; j < Np; ++j) sL += h->lL[j] * F[j][c];
+            for (int j = 0; j < Np; ++j) sR += h->lR[j] * F[j][c];
+            fLc[c] = sL;
+            fRc[c] = sR;
+        }
+        const double Je = h->J[e];
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < Np; ++j) {
+                double vol = 0.0;
+                for (int k = 0; k < Np; ++k) vol += h->D[j + Np * k] * F[k][c];
+                double corr = (E1(h->fL, e, c) - fLc[c]) * h->gLxi[j] + (E1(h->fR, e, c) - fRc[c]) * h->gRxi[j];
+                U1(du, j, e, c) = -(vol + corr) / Je;
+            }
+    }
+    if (h->pb.capturing == ORC_CAPT_NULL) return;
+    orc1_sense(h, u, h->sig);
+    orc1_element_viscosities(h, u, h->sig, h->eps);
+    if (h->pb.capturing == ORC_CAPT_PERSSON_DD) {   /* apply_dissipation_local_DD! (PerssonAV.jl:194-227) */
+        double v[16];
+        for (int e = 0; e < Nel; ++e) {
+            double ee = h->eps[e];
+            if (ee <= 0.0) continue;
+            double Jj = h->J[e];
+            double scale = ee / (Jj * Jj);
+            for (int c = 0; c < 3; ++c) {
+                for (int j = 0; j < Np; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < Np; ++k) s += h->D[j + Np * k] * U1(u, k, e, c);
+                    v[j] = s;
+                }
+                for (int j = 0; j < Np; ++j) {
+                    double s = 0.0;
+                    for (int k = 0; k < Np; ++k) s += h->D[j + Np * k] * v[k];
+                    U1(du, j, e, c) += scale * s;
+                }
+            }
+        }
+        return;
+    }
+    /* apply_dissipation_br0! (PerssonAV.jl:236-340) */
+    double* G = h->g;
+    for (int e = 0; e < Nel; ++e) {
+        double Jj = h->J[e], ee = h->eps[e];
+        for (int c = 0; c < 3; ++c) {
+            for (int j = 0; j < Np; ++j) {
+                double dq = 0.0;
+                for (int k = 0; k < Np; ++k) dq += h->D[j + Np * k] * U1(u, k, e, c);
+                U1(G, j, e, c) = ee * dq / Jj;
+            }
+            double suL = 0.0, suR = 0.0, sgL = 0.0, sgR = 0.0;
+            for (int j = 0; j < Np; ++j) {
+                suL += h->lL[j] * U1(u, j, e, c);
+                suR += h->lR[j] * U1(u, j, e, c);
+                sgL += h->lL[j] * U1(G, j, e, c);
+                sgR += h->lR[j] * U1(G, j, e, c);
+            }
+            E1(h->uL, e, c) = suL; E1(h->uR, e, c) = suR;
+            E1(h->gL, e, c) = sgL; E1(h->gR, e, c) = sgR;
+        }
+    }
+    const int p = h->pb.p;
+#define BR0F(el, er, c) (0.5 * (E1(h->gR, el, c) + E1(h->gL, er, c)) - \
+    (2.0 * (0.5 * (h->eps[el] + h->eps[er])) * (double)(p + 1) / jmax(0.5 * (h->dx[el] + h->dx[er]), EPS)) * \
+    (E1(h->uL, er, c) - E1(h->uR, el, c)))
+    for (int e = 0; e < Nel - 1; ++e)
+        for (int c = 0; c < 3; ++c) {
+            double gh = BR0F(e, e + 1, c);
+            E1(h->ghR, e, c) = gh;
+            E1(h->ghL, e + 1, c) = gh;
+        }
+    if (periodic) {
+        for (int c = 0; c < 3; ++c) {
+            double gh = BR0F(Nel - 1, 0, c);
+            E1(h->ghR, Nel - 1, c) = gh;
+            E1(h->ghL, 0, c) = gh;
+        }
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            E1(h->ghL, 0, c) = E1(h->gL, 0, c);
+            E1(h->ghR, Nel - 1, c) = E1(h->gR, Nel - 1, c);
+        }
+    }
+#undef BR0F
+    for (int e = 0; e < Nel; ++e) {
+        double Jj = h->J[e];
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < Np; ++j) {
+                double vol = 0.0;
+                for (int k = 0; k < Np; ++k) vol += h->D[j + Np * k] * U1(G, k, e, c);
+                double corr = (E1(h->ghL, e, c) - E1(h->gL, e, c)) * h->gLxi[j] +
+                              (E1(h->ghR, e, c) - E1(h->gR, e, c)) * h->gRxi[j];
+                U1(du, j, e, c) += (vol + corr) / Jj;
+            }
+    }
+}
+
+/* compute_dt 1D (SSP_RK3.jl:8-18) with max_wave_speed (Euler.jl:111-120) */
+double orc1_compute_dt(orc1_ctx* h, const double* u, double cfl) {
+    const int Np = h->Np, Nel = h->Nel;
+    const double g = h->pb.gamma;
+    double m = 0.0;
+    for (int e = 0; e < Nel; ++e)
+        for (int j = 0; j < Np; ++j) {
+            double r = U1(u, j, e, 0), ru = U1(u, j, e, 1), E = U1(u, j, e, 2);
+            m = jmax(m, fabs(v1(r, ru)) + c1(g, r, ru, E));
+        }
+    double lam = jmax(jmax(m, EPS), EPS);
+    double dt = INFINITY;
+    for (int e = 0; e < Nel; ++e) dt = jmin(dt, cfl * h->dx[e] / ((double)(2 * h->pb.p + 1) * lam));
+    return dt;
+}
+
+static int nonfinite1(const double* a, size_t n) {
+    for (size_t k = 0; k < n; ++k)
+        if (!isfinite(a[k])) return 1;
+    return 0;
+}
+
+/* ssp_rk3_step! 1D (SSP_RK3.jl:30-62) */
+int orc1_rk3_step(orc1_ctx* h, double* u, double* t, double dt) {
+    const size_t n = (size_t)h->Np * h->Nel * NEQ1;
+    double* u0 = (double*)malloc(8 * n);
+    double* u1 = (double*)malloc(8 * n);
+    double* du = (double*)malloc(8 * n);
+    memcpy(u0, u, 8 * n);
+    const double t0 = *t;
+    int st = ORC_OK;
+    orc1_residual(h, u, *t, du);
+    if (nonfinite1(du, n)) { st = ORC_DIVERGED; goto done; }
+    for (size_t k = 0; k < n; ++k) u[k] = u0[k] + dt * du[k];
+    *t = t0 + dt;
+    orc1_residual(h, u, *t, du);
+    if (nonfinite1(du, n)) { st = ORC_DIVERGED; goto done; }
+    memcpy(u1, u, 8 * n);
+    for (size_t k = 0; k < n; ++k) u[k] = (3.0 / 4.0) * u0[k] + (1.0 / 4.0) * u1[k] + ((1.0 / 4.0) * dt) * du[k];
+    *t = t0 + dt / 2.0;
+    orc1_residual(h, u, *t, du);
+    if (nonfinite1(du, n)) { st = ORC_DIVERGED; goto done; }
+    memcpy(u1, u, 8 * n);
+    for (size_t k = 0; k < n; ++k) u[k] = (1.0 / 3.0) * u0[k] + (2.0 / 3.0) * u1[k] + ((2.0 / 3.0) * dt) * du[k];
+    *t = t0 + dt;
+    if (nonfinite1(u, n)) st = ORC_DIVERGED;
+done:
+    free(u0); free(u1); free(du);
+    return st;
+}
+
+/* ssp_rk3! 1D (SSP_RK3.jl:73-118) */
+int orc1_run(orc1_ctx* h, double* u, double* t, double t_final, double cfl, double fixed_dt,
+             int64_t max_steps, int64_t* n_steps) {
+    int64_t n = 0;
+    int st = ORC_OK;
+    while (*t < t_final - 10.0 * EPS) {
+        double step = fixed_dt > 0.0 ? fixed_dt : orc1_compute_dt(h, u, cfl);
+        if (*t + step > t_final) step = t_final - *t;
+        st = orc1_rk3_step(h, u, t, step);
+        n += 1;
+        if (st != ORC_OK) break;
+        if (n >= max_steps) { st = ORC_MAX_STEPS; break; }
+    }
+    *n_steps = n;
+    return st;
+}
+#undef U1
+#undef E1

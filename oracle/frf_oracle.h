@@ -75,6 +75,30 @@ int orc_rk3_step(orc_ctx* h, double* u, double* t, double dt);
 /* ssp_rk3! (SSP_RK3.jl:166-211). fixed_dt <= 0 selects CFL dt. */
 int orc_run(orc_ctx* h, double* u, double* t, double t_final, double cfl, double fixed_dt,
             int64_t max_steps, int64_t* n_steps);
+/* ---- 1D Euler (Euler1D; the Sod tube configuration) -------------------------------------
+ * u[j,e,c] -> u[j + Np*(e + Nel*c)], Neq = 3 (rho, rho u, E).  Rusanov flux; capturing
+ * ORC_CAPT_NULL, ORC_CAPT_PERSSON_BR0 (conservative_br0) or ORC_CAPT_PERSSON_DD (element-local). */
+typedef struct {
+    int32_t p, nel;
+    int32_t bc_left, bc_right;   /* ORC_BC_PERIODIC (both) or ORC_BC_TRANSMISSIVE */
+    int32_t capturing;
+    int32_t reserved;
+    double gamma, kappa, s0_factor, eps_floor, c_av;
+} orc1_problem;
+typedef struct orc1_ctx orc1_ctx;
+/* xv: nel+1 vertices (Mesh1D.jl:35-48) */
+orc1_ctx* orc1_create(const orc1_problem* pb, const double* xi, const double* w, const double* D,
+                      const double* lL, const double* lR, const double* gLxi, const double* gRxi,
+                      const double* V, const double* xv);
+void orc1_destroy(orc1_ctx* h);
+void orc1_residual(orc1_ctx* h, const double* u, double t, double* du);   /* Residual.jl:161-214 */
+void orc1_sense(orc1_ctx* h, const double* u, double* sigma);             /* PerssonAV.jl:116-135 */
+void orc1_element_viscosities(orc1_ctx* h, const double* u, const double* sigma, double* eps);
+double orc1_compute_dt(orc1_ctx* h, const double* u, double cfl);         /* SSP_RK3.jl:8-18 */
+int orc1_rk3_step(orc1_ctx* h, double* u, double* t, double dt);         /* SSP_RK3.jl:30-62 */
+int orc1_run(orc1_ctx* h, double* u, double* t, double t_final, double cfl, double fixed_dt,
+             int64_t max_steps, int64_t* n_steps);                          /* SSP_RK3.jl:73-118 */
+
 /* Point physics, for unit tests: Rusanov (Euler2D.jl:114-151) and HLLC (HLLC.jl:235-305). */
 void orc_flux_n(int kind, double gamma, const double* uL, const double* uR, double nx, double ny,
                 double* out);

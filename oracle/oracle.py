@@ -27,6 +27,13 @@ class OrcProblem(C.Structure):
                 ("s0_factor", C.c_double), ("eps_floor", C.c_double), ("c_av", C.c_double)]
 
 
+class Orc1Problem(C.Structure):
+    _fields_ = [("p", C.c_int32), ("nel", C.c_int32), ("bc_left", C.c_int32),
+                ("bc_right", C.c_int32), ("capturing", C.c_int32), ("reserved", C.c_int32),
+                ("gamma", C.c_double), ("kappa", C.c_double), ("s0_factor", C.c_double),
+                ("eps_floor", C.c_double), ("c_av", C.c_double)]
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
     if force or not os.path.exists(_LIB_PATH):
@@ -67,6 +74,19 @@ def lib():
         L.orc_run.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double,
                               C.c_int64, C.POINTER(C.c_int64)]
         L.orc_flux_n.argtypes = [C.c_int, C.c_double, dp, dp, C.c_double, C.c_double, dp]
+        L.orc1_create.restype = C.c_void_p
+        L.orc1_create.argtypes = [C.POINTER(Orc1Problem)] + [dp] * 9
+        L.orc1_destroy.argtypes = [C.c_void_p]
+        L.orc1_residual.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.orc1_sense.argtypes = [C.c_void_p, dp, dp]
+        L.orc1_element_viscosities.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc1_compute_dt.restype = C.c_double
+        L.orc1_compute_dt.argtypes = [C.c_void_p, dp, C.c_double]
+        L.orc1_rk3_step.restype = C.c_int
+        L.orc1_rk3_step.argtypes = [C.c_void_p, dp, dp, C.c_double]
+        L.orc1_run.restype = C.c_int
+        L.orc1_run.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double,
+                               C.c_int64, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -196,4 +216,69 @@ class Oracle:
         n = C.c_int64(0)
         st = lib().orc_run(self.h, _p(u), C.byref(tt), t_final, cfl, -1.0 if dt is None else dt,
                            max_steps, C.byref(n))
+        return STATUS[st], n.value, tt.value, u
+
+
+class Oracle1D:
+    """The 1D Euler restatement (Sod tube configuration): u[j, e, c] with shape (Np, Nel, 3)."""
+
+    def __init__(self, p, nel, xv, ops, bc="transmissive", capturing="null", gamma=1.4, kappa=4.0,
+                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1):
+        b = BC_PERIODIC if bc == "periodic" else BC_TRANSMISSIVE
+        pb = Orc1Problem(p, nel, b, b, CAPT[capturing], 0, gamma, kappa, s0_factor, eps_floor, c_av)
+        self.p, self.nel, self.Np = p, nel, p + 1
+        self._keep = {k: np.asfortranarray(np.asarray(ops[k], dtype=np.float64))
+                      for k in ("xi", "w", "D", "lL", "lR", "gLxi", "gRxi", "V")}
+        self._xv = np.ascontiguousarray(xv, dtype=np.float64)
+        k = self._keep
+        self.h = lib().orc1_create(C.byref(pb), _p(k["xi"]), _p(k["w"]), _p(k["D"]), _p(k["lL"]),
+                                   _p(k["lR"]), _p(k["gLxi"]), _p(k["gRxi"]), _p(k["V"]), _p(self._xv))
+        if not self.h:
+            raise ValueError("orc1_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc1_destroy(self.h)
+            self.h = None
+
+    def _state(self, u):
+        u = np.asfortranarray(u, dtype=np.float64)
+        assert u.shape == (self.Np, self.nel, 3), u.shape
+        return u
+
+    def residual(self, u, t=0.0):
+        u = self._state(u)
+        du = np.zeros_like(u, order="F")
+        lib().orc1_residual(self.h, _p(u), t, _p(du))
+        return du
+
+    def sense(self, u):
+        u = self._state(u)
+        s = np.zeros(self.nel)
+        lib().orc1_sense(self.h, _p(u), _p(s))
+        return s
+
+    def element_viscosities(self, u, sigma):
+        u = self._state(u)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        e = np.zeros(self.nel)
+        lib().orc1_element_viscosities(self.h, _p(u), _p(sg), _p(e))
+        return e
+
+    def compute_dt(self, u, cfl):
+        return lib().orc1_compute_dt(self.h, _p(self._state(u)), cfl)
+
+    def rk3_step(self, u, t, dt):
+        u = self._state(u).copy(order="F")
+        tt = C.c_double(t)
+        st = lib().orc1_rk3_step(self.h, _p(u), C.byref(tt), dt)
+        return STATUS[st], u, tt.value
+
+    def run(self, u, t, t_final, cfl=0.2, dt=None, max_steps=10**7):
+        """ssp_rk3! 1D; returns (status, n_steps, t, u_new)."""
+        u = self._state(u).copy(order="F")
+        tt = C.c_double(t)
+        n = C.c_int64(0)
+        st = lib().orc1_run(self.h, _p(u), C.byref(tt), t_final, cfl, -1.0 if dt is None else dt,
+                            max_steps, C.byref(n))
         return STATUS[st], n.value, tt.value, u

@@ -21,7 +21,7 @@ OBJDIR = os.path.join(HERE, "_build")
 ARCH = os.environ.get("FRF_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["frf_stage.hip", "frf_stage_mixed.hip", "frf_batch_kernels.hip", "frf_kernels.hip",
-           "frf_capi.hip"]
+           "frf_capi.hip", "frf_1d.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

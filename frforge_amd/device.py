@@ -27,7 +27,10 @@ EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_s
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
             "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy",
             "frf_batch_create", "frf_batch_tune", "frf_batch_run", "frf_batch_tile",
-            "frf_batch_last_error", "frf_batch_destroy")
+            "frf_batch_last_error", "frf_batch_destroy",
+            "frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_upload_state",
+            "frf1d_download_state", "frf1d_residual", "frf1d_rk3_step", "frf1d_run",
+            "frf1d_compute_dt", "frf1d_last_error", "frf1d_destroy")
 
 
 class FrfProblem(C.Structure):
@@ -37,6 +40,13 @@ class FrfProblem(C.Structure):
                 ("capturing", C.c_int32), ("reserved", C.c_int32), ("gamma", C.c_double),
                 ("kappa", C.c_double), ("s0_factor", C.c_double), ("eps_floor", C.c_double),
                 ("c_av", C.c_double)]
+
+
+class Frf1dProblem(C.Structure):
+    _fields_ = [("p", C.c_int32), ("nel", C.c_int32), ("neq", C.c_int32), ("bc_left", C.c_int32),
+                ("bc_right", C.c_int32), ("flux", C.c_int32), ("precision", C.c_int32),
+                ("capturing", C.c_int32), ("gamma", C.c_double), ("kappa", C.c_double),
+                ("s0_factor", C.c_double), ("eps_floor", C.c_double), ("c_av", C.c_double)]
 
 
 class FrfRunResult(C.Structure):
@@ -99,6 +109,22 @@ def load_library(path: str = LIB_PATH):
     L.frf_batch_destroy.argtypes = [vp]
     for name in ("frf_batch_create", "frf_batch_tune", "frf_batch_run", "frf_batch_tile",
                  "frf_batch_destroy"):
+        getattr(L, name).restype = C.c_int32
+    L.frf1d_create.argtypes = [C.POINTER(Frf1dProblem), C.POINTER(vp)]
+    L.frf1d_set_operators.argtypes = [vp] + [dp] * 8
+    L.frf1d_set_geometry.argtypes = [vp, dp]
+    L.frf1d_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
+    L.frf1d_download_state.argtypes = [vp, dp, C.c_int64, dp]
+    L.frf1d_residual.argtypes = [vp, dp, C.c_int64]
+    L.frf1d_rk3_step.argtypes = [vp, C.c_double, C.POINTER(C.c_int32)]
+    L.frf1d_run.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int64, C.POINTER(FrfRunResult)]
+    L.frf1d_compute_dt.argtypes = [vp, C.c_double, dp]
+    L.frf1d_last_error.restype = C.c_char_p
+    L.frf1d_last_error.argtypes = [vp]
+    L.frf1d_destroy.argtypes = [vp]
+    for name in ("frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_upload_state",
+                 "frf1d_download_state", "frf1d_residual", "frf1d_rk3_step", "frf1d_run",
+                 "frf1d_compute_dt", "frf1d_destroy"):
         getattr(L, name).restype = C.c_int32
     for name in ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
                  "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt",
@@ -286,3 +312,80 @@ class DeviceBatch:
             self.close()
         except Exception:
             pass
+
+
+class DeviceContext1D:
+    """frf1d_ctx: a 1D Euler problem (Euler1D, FP64) resident on the GPU."""
+
+    def __init__(self, *, p, nel, bc="transmissive", capturing="null", gamma=1.4, kappa=4.0,
+                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1):
+        self.lib = load_library()
+        pb = Frf1dProblem(p, nel, 3, BC[bc], BC[bc], FLUX["rusanov"], PRECISION["fp64"],
+                          CAPTURING[capturing], gamma, kappa, s0_factor, eps_floor, c_av)
+        h = C.c_void_p()
+        rc = self.lib.frf1d_create(C.byref(pb), C.byref(h))
+        if rc != FRF_OK:
+            raise FrfError(f"frf1d_create: {ERRORS.get(rc, rc)}: {self.lib.frf1d_last_error(None).decode()}")
+        self.h = h
+        self.p, self.nel = p, nel
+        self.nstate = (p + 1) * nel * 3
+
+    def _chk(self, rc, what):
+        if rc != FRF_OK:
+            raise FrfError(f"{what}: {ERRORS.get(rc, rc)}: {self.lib.frf1d_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frf1d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_operators(self, xi, w, D, ell_L, ell_R, gL_xi, gR_xi, V):
+        arrs = [np.asfortranarray(np.asarray(a, dtype=np.float64)) for a in (xi, w, D, ell_L, ell_R, gL_xi, gR_xi, V)]
+        self._ops_keep = arrs
+        self._chk(self.lib.frf1d_set_operators(self.h, *[_dp(a) for a in arrs]), "frf1d_set_operators")
+
+    def set_geometry(self, x_vertices):
+        xv = np.ascontiguousarray(x_vertices, dtype=np.float64)
+        self._chk(self.lib.frf1d_set_geometry(self.h, _dp(xv)), "frf1d_set_geometry")
+
+    def _check(self, u):
+        if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.f_contiguous
+                and u.size == self.nstate):
+            raise ValueError(f"state must be a Fortran-ordered float64 array of {self.nstate} values")
+
+    def upload(self, u, t):
+        self._check(u)
+        self._chk(self.lib.frf1d_upload_state(self.h, _dp(u), self.nstate, float(t)), "frf1d_upload_state")
+
+    def download(self, u) -> float:
+        self._check(u)
+        t = C.c_double()
+        self._chk(self.lib.frf1d_download_state(self.h, _dp(u), self.nstate, C.byref(t)), "frf1d_download_state")
+        return t.value
+
+    def residual(self, du):
+        self._check(du)
+        self._chk(self.lib.frf1d_residual(self.h, _dp(du), self.nstate), "frf1d_residual")
+        return du
+
+    def rk3_step(self, dt) -> str:
+        st = C.c_int32()
+        self._chk(self.lib.frf1d_rk3_step(self.h, float(dt), C.byref(st)), "frf1d_rk3_step")
+        return STATUS[st.value]
+
+    def run(self, t_final, cfl, dt=None, max_steps=10**7):
+        res = FrfRunResult()
+        self._chk(self.lib.frf1d_run(self.h, float(t_final), float(cfl), -1.0 if dt is None else float(dt),
+                                     int(max_steps), C.byref(res)), "frf1d_run")
+        return {"status": STATUS[res.status], "n_steps": res.n_steps, "t": res.t, "fail_stage": res.fail_stage}
+
+    def compute_dt(self, cfl) -> float:
+        dt = C.c_double()
+        self._chk(self.lib.frf1d_compute_dt(self.h, float(cfl), C.byref(dt)), "frf1d_compute_dt")
+        return dt.value

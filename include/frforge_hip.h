@@ -136,6 +136,44 @@ const char* frf_last_error(const frf_ctx* ctx);
 
 int32_t frf_destroy(frf_ctx* ctx);
 
+/* ---- 1D Euler path (the reference's Sod tube configuration): Euler1D, Rusanov flux, Null /
+ * Persson+BR0 / Persson+local-DD capturing, SSP-RK3, FP64 in the reference's operation order.
+ * State u[j,e,c] -> u[j + Np*(e + Nel*c)], Neq = 3 (SolutionState.jl:25-39).  A run is one
+ * device launch (time loop on the device). */
+typedef struct frf1d_problem {
+    int32_t p;           /* 1..5 */
+    int32_t nel;
+    int32_t neq;         /* must be 3 (Euler1D) */
+    int32_t bc_left, bc_right;   /* FRF_BC_PERIODIC (both) or FRF_BC_TRANSMISSIVE */
+    int32_t flux;        /* FRF_FLUX_RUSANOV */
+    int32_t precision;   /* FRF_FP64 */
+    int32_t capturing;   /* FRF_CAPT_* */
+    double gamma;
+    double kappa, s0_factor, eps_floor, c_av;
+} frf1d_problem;
+typedef struct frf1d_ctx frf1d_ctx;
+/* allocate_state(Mesh1D, ...) (SolutionState.jl:25-39) */
+int32_t frf1d_create(const frf1d_problem* problem, frf1d_ctx** out);
+/* build_operators(p) bits (Operators.jl:85-100), matrices column-major */
+int32_t frf1d_set_operators(frf1d_ctx* ctx, const double* xi, const double* w, const double* D,
+                            const double* ell_L, const double* ell_R, const double* gL_xi,
+                            const double* gR_xi, const double* V_legendre);
+/* Mesh1D.x_vertices (nel+1 values; Mesh1D.jl:35-48) */
+int32_t frf1d_set_geometry(frf1d_ctx* ctx, const double* x_vertices);
+int32_t frf1d_upload_state(frf1d_ctx* ctx, const double* u, int64_t len, double t);
+int32_t frf1d_download_state(frf1d_ctx* ctx, double* u, int64_t len, double* t);
+/* residual!(du, state, eq, method) 1D (Residual.jl:161-214) */
+int32_t frf1d_residual(frf1d_ctx* ctx, double* du, int64_t len);
+/* ssp_rk3_step! 1D (SSP_RK3.jl:30-62) */
+int32_t frf1d_rk3_step(frf1d_ctx* ctx, double dt, int32_t* status);
+/* ssp_rk3! 1D (SSP_RK3.jl:73-118), as frf_run */
+int32_t frf1d_run(frf1d_ctx* ctx, double t_final, double cfl, double fixed_dt, int64_t max_steps,
+                  frf_run_result* result);
+/* compute_dt 1D (SSP_RK3.jl:8-18) */
+int32_t frf1d_compute_dt(frf1d_ctx* ctx, double cfl, double* dt);
+const char* frf1d_last_error(const frf1d_ctx* ctx);
+int32_t frf1d_destroy(frf1d_ctx* ctx);
+
 /* ---- Batched form: many independent instances advanced together in one loop.
  * No single reference counterpart: it runs what the reference does as a sweep of independent
  * ssp_rk3! / integrate! calls over meshes, degrees and fluxes (the case runners' loops,

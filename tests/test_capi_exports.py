@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "frforge_hip.h")
 def _declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(frf_[A-Za-z0-9_]+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(frf(?:1d)?_[A-Za-z0-9_]+)\s*\(",
                                  text, flags=re.M)))
 
 
@@ -44,7 +44,7 @@ def test_load_library_fails_loudly_when_missing(tmp_path, monkeypatch):
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
 def test_ctypes_struct_layout_matches_header(tmp_path):
     structs = {"frf_problem": dv.FrfProblem, "frf_run_result": dv.FrfRunResult,
-               "frf_profile": dv.FrfProfile}
+               "frf_profile": dv.FrfProfile, "frf1d_problem": dv.Frf1dProblem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "frforge_hip.h"',
              "int main(void) {"]
     for cname, py in structs.items():

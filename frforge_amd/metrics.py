@@ -1,7 +1,10 @@
-"""Shock-quality metrics of the reference's verification suite (Metrics.jl:75-255), used by the
-Sod case runner: overshoot, smooth-region mask, excess dissipation and 10-90 % shock thickness.
-Inputs are solution-point samples sorted by x."""
+"""Metrics of the reference's verification suite (Metrics.jl): overshoot, smooth-region mask,
+excess dissipation and 10-90 % shock thickness for the discontinuous cases (inputs are
+solution-point samples sorted by x), observed orders and the order gate for the smooth cases,
+conservation residuals and the linear interpolation used against frozen reference data."""
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
@@ -92,3 +95,58 @@ def smooth_region_mask(x, rho, grad_frac: float = 0.1, exclude_windows=()):
             if a <= x[i] <= b:
                 mask[i] = False
     return mask
+
+
+DEFAULT_ORDER_TOLERANCE = 0.3   # Metrics.jl:3
+
+
+def observed_orders(mesh_sizes, errors):
+    """q_i = log(E_i / E_{i+1}) / log(h_i / h_{i+1}); NaN where an error is not positive
+    (Metrics.jl:12-27)."""
+    qs = []
+    for i in range(len(errors) - 1):
+        h0, h1 = mesh_sizes[i], mesh_sizes[i + 1]
+        e0, e1 = errors[i], errors[i + 1]
+        if not (e0 > 0 and e1 > 0):
+            qs.append(float("nan"))
+        else:
+            qs.append(math.log(e0 / e1) / math.log(h0 / h1))
+    return qs
+
+
+def order_pass(observed, formal_order, tol: float = DEFAULT_ORDER_TOLERANCE) -> bool:
+    """Every observed order finite and at least formal_order - tol; False when there is none
+    (Metrics.jl:34-46)."""
+    if len(observed) == 0:
+        return False
+    return all(math.isfinite(q) and not q < formal_order - tol for q in observed)
+
+
+def conservation_residual_relative(m0, mt) -> float:
+    """|M(T) - M(0)| / max(|M0|, |MT|, 1) (Metrics.jl:54-58)."""
+    return abs(mt - m0) / max(abs(m0), abs(mt), 1.0)
+
+
+def conservation_residual_absolute(m0, mt) -> float:
+    """|M(T) - M(0)| (Metrics.jl:61)."""
+    return abs(mt - m0)
+
+
+def interp_1d(x_ref, v_ref, xq) -> np.ndarray:
+    """Piecewise-linear interpolation of (x_ref, v_ref) at xq, held constant outside the range
+    (Metrics.jl:258-275): j is the last index with x_ref[j] <= x."""
+    x_ref = np.asarray(x_ref, dtype=np.float64)
+    v_ref = np.asarray(v_ref, dtype=np.float64)
+    out = np.empty(len(xq))
+    n = len(x_ref)
+    for i, x in enumerate(np.asarray(xq, dtype=np.float64)):
+        if x <= x_ref[0]:
+            out[i] = v_ref[0]
+        elif x >= x_ref[-1]:
+            out[i] = v_ref[-1]
+        else:
+            j = int(np.searchsorted(x_ref, x, side="right")) - 1
+            j = min(max(j, 0), n - 2)
+            t = (x - x_ref[j]) / (x_ref[j + 1] - x_ref[j] + 1e-30)
+            out[i] = (1 - t) * v_ref[j] + t * v_ref[j + 1]
+    return out

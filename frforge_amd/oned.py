@@ -119,6 +119,13 @@ def discrete_mass_1d(state: SolutionState1D, c: int = 0) -> float:
     return m
 
 
+def positivity_ok_1d(eq: Euler1D, u: np.ndarray, atol: float = 0.0) -> bool:
+    """rho > atol, p > atol and both finite at every solution point (Euler.jl:62-77)."""
+    rho = u[:, :, 0]
+    p = eq.pressure(u)
+    return bool(np.all((rho > atol) & (p > atol) & np.isfinite(rho) & np.isfinite(p)))
+
+
 def solution_extrema(state: SolutionState1D, c: int = 0):
     """(min, max) of component c over all solution points (Metrics.jl:64-73)."""
     v = state.u[:, :, c]

@@ -1,34 +1,40 @@
 """The Sod shock-tube case runner (run_sod, Cases.jl:951-1049): integrate with a capturing
 method and with NullCapturing, then score density against the exact solution and against the
 NullCapturing run.  The time loop runs on the device (frforge_amd.solver1d.ssp_rk3_1d) unless an
-``integrator(state, eq, method, t_final, cfl) -> {"status", "n_steps", "t"}`` is given."""
+``integrator(state, eq, method, t_final, cfl, dt=None) -> {"status", "n_steps", "t"}`` is given
+(dt fixes the step; the last step is shortened to land on t_final)."""
 from __future__ import annotations
+
+import time
 
 import numpy as np
 
-from .capturing import NullCapturing, PerssonAVMethod
+from .capturing import NullCapturing, PerssonAVMethod, method_params
 from .exact_sod import RiemannProblem1D, sample
 from .mesh import TransmissiveBC
 from .metrics import excess_dissipation, overshoot_metric, shock_thickness_sp, smooth_region_mask
-from .oned import (Euler1D, Mesh1D, allocate_state_1d, discrete_mass_1d, sample_solution_1d,
-                   set_initial_condition_1d, sod_ic, solution_extrema)
+from .oned import (Euler1D, Mesh1D, allocate_state_1d, discrete_mass_1d, positivity_ok_1d,
+                   sample_solution_1d, set_initial_condition_1d, sod_ic, solution_extrema)
 from .operators import build_operators
 from .state import SchemeConfig
 
 EXCLUDE = [(0.45, 0.55), (0.65, 0.95)]   # contact and shock neighbourhoods at t = 0.2
 
 
-def _default_integrator(state, eq, method, t_final, cfl):
+def device_integrator(state, eq, method, t_final, cfl, dt=None):
+    """integrate! on the device (SSP_RK3.jl:73-118)."""
     from .solver1d import ssp_rk3_1d
-    return ssp_rk3_1d(state, eq, method, t_final, cfl=cfl)
+    return ssp_rk3_1d(state, eq, method, t_final, cfl=cfl, dt=dt)
 
 
 def run_sod(p: int = 2, n_elements: int = 64, t_final: float = 0.2, cfl: float = 0.2,
             gamma: float = 1.4, method=None, method_name: str = "persson_av",
             scheme: SchemeConfig = None, integrator=None) -> dict:
-    """Returns the case report fields of the reference (Cases.jl:1008-1048)."""
+    """The case report of the reference's run_sod (Cases.jl:951-1049)."""
+    from .report import case_report_dict
+    t0 = time.time()
     method = method if method is not None else PerssonAVMethod.create()
-    integrator = integrator or _default_integrator
+    integrator = integrator or device_integrator
     scheme = scheme or SchemeConfig()
     eq = Euler1D(gamma)
     ops = build_operators(p)
@@ -41,7 +47,7 @@ def run_sod(p: int = 2, n_elements: int = 64, t_final: float = 0.2, cfl: float =
     result = integrator(state, eq, method, t_final, cfl_run)
     diverged = result["status"] != "ok"
     nan = diverged or not np.all(np.isfinite(state.u))
-    pos = (not nan) and bool(np.all(state.u[:, :, 0] > 0) and np.all(eq.pressure(state.u) > 0))
+    pos = (not nan) and positivity_ok_1d(eq, state.u)
     mt = discrete_mass_1d(state, 0)
     x, rho = sample_solution_1d(state, 0)
     umin, umax = solution_extrema(state, 0)
@@ -61,15 +67,15 @@ def run_sod(p: int = 2, n_elements: int = 64, t_final: float = 0.2, cfl: float =
     dex = dex_null if dex_null is not None else dex_exact
     delta = shock_thickness_sp(x, rho)
     l1 = float(np.sum(np.abs(rho - rho_ex)) / (np.sum(np.abs(rho_ex)) + 1e-30))
-    return {
-        "name": f"sod_p{p}_ne{n_elements}_{method_name}", "case_type": "discontinuous",
-        "equation": "euler1d", "p": p, "capturing_method": method_name,
-        "pass": (not diverged) and (not nan) and pos, "diverged": diverged, "nan_detected": nan,
-        "conservation_residual": abs(mt - m0), "conservation_pass": True,
-        "conservation_metric": "none", "positivity_ok": pos, "n_elements": n_elements,
-        "t_final": t_final, "excess_dissipation": dex, "shock_thickness": delta,
-        "shock_thickness_unit": "sp_spacings", "overshoot": eta,
-        "metrics": {"excess_dissipation_vs_null": dex_null, "excess_dissipation_vs_exact": dex_exact,
-                    "l1_density_vs_exact": l1, "mass_change": mt - m0,
-                    "n_steps": result["n_steps"], "u_min": umin, "u_max": umax},
-    }
+    return case_report_dict(
+        name=f"sod_p{p}_ne{n_elements}_{method_name}", case_type="discontinuous",
+        equation="euler1d", p=p, capturing_method=method_name,
+        passed=(not diverged) and (not nan) and pos, diverged=diverged, nan_detected=nan,
+        conservation_residual=abs(mt - m0), conservation_pass=True, conservation_metric="none",
+        positivity_ok=pos, wall_time_sec=time.time() - t0, n_elements=n_elements,
+        t_final=t_final, excess_dissipation=dex, shock_thickness=delta,
+        shock_thickness_unit="sp_spacings", overshoot=eta,
+        metrics={"excess_dissipation_vs_null": dex_null, "excess_dissipation_vs_exact": dex_exact,
+                 "l1_density_vs_exact": l1, "mass_change": mt - m0,
+                 "n_steps": result["n_steps"], "u_min": umin, "u_max": umax},
+        extra={"l1_error_vs_reference": l1, "method_params": method_params(method)})

@@ -59,3 +59,21 @@ def rel_err_per_component(a, b):
         den = max(np.max(np.abs(b[:, :, :, c])), 1e-300)
         out.append(np.max(np.abs(a[:, :, :, c] - b[:, :, :, c])) / den)
     return max(out)
+
+
+def oracle_for_state(state, method, eq=EQ):
+    """The oracle problem matching a 2D state (mesh, BCs, flux) and a capturing method."""
+    kind, prm = method.device_capturing()
+    bx = "periodic" if isinstance(state.mesh.left_bc, PeriodicBC) else "transmissive"
+    by = "periodic" if isinstance(state.mesh.bottom_bc, PeriodicBC) else "transmissive"
+    m = state.mesh
+    return Oracle(state.p, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(state.ops), bc_x=bx,
+                  bc_y=by, flux=state.scheme.flux, capturing=kind, gamma=eq.gamma, **prm)
+
+
+def oracle_integrator_2d(state, eq, method, t_final, cfl, dt=None):
+    """integrate! through the oracle's time loop, writing the result back into the state."""
+    st, n, t, u = oracle_for_state(state, method, eq).run(state.u, state.t, t_final, cfl=cfl, dt=dt)
+    state.u[...] = u
+    state.t = t
+    return {"status": st, "n_steps": n, "t": t}

@@ -225,3 +225,25 @@ def test_max_steps_status():
     st_o, n_o, t_o, u_o = orc.run(make_case(2, 8, 8, ic="vortex")[0].u, 0.0, 10.0, cfl=0.1, max_steps=3)
     assert (r["status"], r["n_steps"]) == ("max_steps", 3) == (st_o, n_o)
     assert r["t"] == t_o
+
+
+@pytest.mark.parametrize("capturing", ["null", "persson_br0", "persson_dd"])
+@pytest.mark.parametrize("bc", ["periodic", "transmissive"])
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_mixed_uniform_state_is_exact(p, bc, capturing):
+    """A uniform moving state has a zero residual bit for bit in the FP32-mixed form (Rusanov),
+    and a run keeps it uniform up to the FP64 stage combine's own rounding (the 1/3, 2/3 blend):
+    no single-precision rounding source in uniform regions."""
+    from frforge_amd.equations import primitives_to_conserved
+    state, method, _ = make_case(p, 37, 21, bc=bc, capturing=capturing, ic="vortex",
+                                 precision="fp32_mixed")
+    U = primitives_to_conserved(EQ, 1.7867813345168273, -0.4386150445627197, 0.2865673511062823,
+                                0.319908645515941)
+    state.u[...] = np.asarray(U)[None, None, None, :]
+    du = np.ones(state.u.shape, order="F")
+    residual(du, state, EQ, method)
+    assert np.all(du == 0.0), np.max(np.abs(du))
+    u0 = state.u.copy()
+    r = ssp_rk3(state, EQ, method, 0.01, cfl=0.1)
+    assert r["status"] == "ok"
+    assert np.max(np.abs(state.u - u0)) <= 1e-14 * np.max(np.abs(u0))

@@ -1,0 +1,165 @@
+"""The scoring harness and the 1D quant suite, pinned by the reference's snapshot fixture
+(test/data/snapshot_fixture/{baseline,method,compare}.json, copied to tests/golden): the
+absolute scores of each report recomputed from its own cases, the relative scores and the
+candidate classification of compare.json, and the whole quant suite rerun through the oracle's
+time loop, which must reproduce every case and both composites."""
+import copy
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from frforge_amd.metrics import interp_1d, observed_orders, order_pass
+from frforge_amd.quant import quant_report
+from frforge_amd.report import report_skeleton, validate_report_keys, write_report, load_report
+from frforge_amd.scoring import (classify_candidate, collect_hard_gate_failures,
+                                 score_suite_absolute, score_suite_relative, tradeoff_ok)
+
+from test_oracle1d import _oracle_for
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SHU_CSV = os.path.join(GOLD, "shu_osher_ref.csv")
+
+
+def _fixture(name):
+    with open(os.path.join(GOLD, "snapshot_fixture", name + ".json")) as f:
+        return json.load(f)
+
+
+BASE, METHOD, COMPARE = _fixture("baseline"), _fixture("method"), _fixture("compare")
+
+
+def _close_scores(got, want, rtol):
+    assert set(got) == set(want)
+    for k in want:
+        assert math.isclose(got[k], want[k], rel_tol=rtol, abs_tol=0.0), (k, got[k], want[k])
+
+
+@pytest.mark.parametrize("rep", [BASE, METHOD], ids=["persson_av", "scaled_persson"])
+def test_absolute_scores_reproduce_fixture(rep):
+    _close_scores(score_suite_absolute(rep["cases"]), rep["summary"]["scores"], 1e-15)
+
+
+def test_relative_scores_and_classification_reproduce_compare():
+    _close_scores(score_suite_relative(METHOD["cases"], BASE["cases"]), COMPARE["relative_scores"], 1e-15)
+    cmp = classify_candidate(METHOD, BASE)
+    for k in ("candidate_status", "baseline_name", "tradeoff_ok", "tradeoff_notes", "vtk_produced",
+              "score_margin_threshold"):
+        assert cmp[k] == COMPARE[k], k
+    assert math.isclose(cmp["composite_margin"], COMPARE["composite_margin"], rel_tol=1e-14)
+    assert cmp["candidate_status"] == METHOD["candidate_status"]
+
+
+def test_classification_thresholds():
+    """A margin at delta with a trade-off win is promising (accepted with VTK output); a failed
+    report is rejected whatever its score."""
+    m = copy.deepcopy(METHOD)
+    assert classify_candidate(m, BASE, delta=0.004)["candidate_status"] == "promising"
+    assert classify_candidate(m, BASE, delta=0.004, vtk_produced=True)["candidate_status"] == "accepted_candidate"
+    m["overall_pass"] = False
+    assert classify_candidate(m, BASE, delta=0.0)["candidate_status"] == "rejected"
+    ok, notes = tradeoff_ok({**BASE["summary"]["scores"], "order_preservation": 0.0}, BASE["summary"]["scores"])
+    assert not ok and notes == "order_regressed"
+
+
+def test_robustness_and_hard_gates():
+    cases = copy.deepcopy(BASE["cases"])
+    assert collect_hard_gate_failures(cases) == []
+    cases[0]["order_pass"] = False
+    cases[2]["diverged"] = True
+    cases[3]["positivity_ok"] = False
+    s = score_suite_absolute(cases)
+    assert s["order_preservation"] == 0.0 and s["robustness"] == 0.0
+    assert collect_hard_gate_failures(cases) == [
+        "euler_smooth_order_p2 order_pass failed", "sod_p2_ne64_persson_av diverged/nan",
+        "shu_osher_p1_ne100_persson_av positivity failed"]
+
+
+def test_absolute_maps_closed_form():
+    """One discontinuous case: diss = 1 - Dex/0.2, shock = (1 - delta/8)/2 + (1 - eta/0.1)/2,
+    missing Dex falls back to the overshoot map, an infinite thickness scores 1."""
+    c = {"case_type": "discontinuous", "excess_dissipation": 0.05, "overshoot": 0.02, "shock_thickness": 2.0}
+    s = score_suite_absolute([c])
+    assert math.isclose(s["dissipation"], 0.75) and math.isclose(s["shock_quality"], 0.5 * 0.75 + 0.5 * 0.8)
+    assert math.isclose(s["composite"], 0.3 + 0.25 * 0.75 + 0.25 * 0.775 + 0.2)
+    c2 = {"case_type": "discontinuous", "overshoot": 0.03, "shock_thickness": float("inf")}
+    s2 = score_suite_absolute([c2])
+    assert math.isclose(s2["dissipation"], 0.7) and math.isclose(s2["shock_quality"], 0.5 + 0.5 * 0.7)
+    assert score_suite_absolute([])["composite"] == 1.0
+
+
+def test_report_schema_validation(tmp_path):
+    assert validate_report_keys(BASE) == [] and validate_report_keys(METHOD) == []
+    bad = copy.deepcopy(BASE)
+    del bad["summary"]["scores"]["robustness"]
+    del bad["cases"][1]["metrics"]
+    bad["overall_score"] = 1.0
+    errs = validate_report_keys(bad)
+    assert "missing summary.scores key: robustness" in errs
+    assert "cases[2] missing key: metrics" in errs
+    assert "overall_score is forbidden; use summary.scores.composite only" in errs
+    rep = report_skeleton(command="invent", suite="quant", method_name="persson_av", cases=BASE["cases"])
+    assert rep["summary"]["scores"] == score_suite_absolute(BASE["cases"])
+    path = str(tmp_path / "r.json")
+    write_report(path, rep)
+    assert validate_report_keys(load_report(path)) == []
+
+
+def test_order_metrics_closed_forms():
+    h = [0.1, 0.05, 0.025]
+    assert np.allclose(observed_orders(h, [3 * x ** 4 for x in h]), [4.0, 4.0], rtol=1e-14)
+    assert math.isnan(observed_orders(h, [1.0, 0.0, 1.0])[0])
+    assert order_pass([3.75, 3.8], 4) and not order_pass([3.65], 4) and not order_pass([], 4)
+    xr = np.array([0.0, 1.0, 3.0])
+    vr = np.array([1.0, 3.0, -1.0])
+    np.testing.assert_allclose(interp_1d(xr, vr, [-1.0, 0.5, 1.0, 2.0, 5.0]), [1.0, 2.0, 3.0, 1.0, -1.0])
+
+
+def oracle_integrator(state, eq, method, t_final, cfl, dt=None):
+    st, n, t, u = _oracle_for(state, eq, method).run(state.u, state.t, t_final, cfl=cfl, dt=dt)
+    state.u[...] = u
+    state.t = t
+    return {"status": st, "n_steps": n, "t": t}
+
+
+# Tolerances against the fixture: the smooth-order L2 errors are differences of nearly equal
+# numbers after 10^4 steps (measured rel. differences up to 2.6e-9); the conservation residuals
+# are roundoff noise (~1e-13) and are only checked against their gates.
+RTOL = {"l2_errors": 1e-8, "observed_orders": 1e-8, "default": 1e-9}
+NOISE = {"conservation_residual", "conservation_residual_components", "mass_abs_change_components",
+         "mass_change", "wall_time_sec"}
+
+
+def assert_case_matches(got, want):
+    for k, w in want.items():
+        if k in NOISE:
+            continue
+        g = got[k]
+        if isinstance(w, dict):
+            assert_case_matches(g, w)
+        elif isinstance(w, list):
+            tol = RTOL.get(k, RTOL["default"])
+            assert len(g) == len(w), k
+            for a, b in zip(g, w):
+                assert (a == b) if not isinstance(b, float) else math.isclose(a, b, rel_tol=tol), (k, a, b)
+        elif isinstance(w, float):
+            assert math.isclose(g, w, rel_tol=RTOL["default"]), (k, g, w)
+        else:
+            assert g == w, (k, g, w)
+
+
+def check_quant_report(rep, fixture):
+    assert [c["name"] for c in rep["cases"]] == [c["name"] for c in fixture["cases"]]
+    for c, f in zip(rep["cases"], fixture["cases"]):
+        assert_case_matches(c, f)
+    assert rep["overall_pass"] is True and rep["hard_gate_failures"] == []
+    _close_scores(rep["summary"]["scores"], fixture["summary"]["scores"], 1e-10)
+    assert validate_report_keys(rep) == []
+
+
+@pytest.mark.parametrize("name,fixture", [("persson_av", BASE), ("scaled_persson", METHOD)])
+def test_quant_suite_through_oracle_reproduces_fixture(name, fixture):
+    rep = quant_report(name, integrator=oracle_integrator, shu_osher_reference=SHU_CSV)
+    check_quant_report(rep, fixture)

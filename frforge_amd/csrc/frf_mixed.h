@@ -55,6 +55,62 @@
 
 namespace frf {
 
+// ---- Line operators that are exact on constant lines.
+// With d_k = v_k - v_0 a Lagrange trace (weights summing to one) is v_0 + sum_{k>=1} w_k d_k
+// and a derivative row (weights summing to zero) is sum_{k>=1} w_k d_k.  The two forms agree
+// with sum_k w_k v_k in exact arithmetic; in single precision the plain sums leave a rounding
+// residue of order eps_32 |v| |w| on a constant line, which the stage turns into a spurious
+// source of eps_32 |F| |D| / J in a uniform region, while here every d_k of a constant line is
+// exactly zero and the residual of a uniform state vanishes bit for bit, as in FP64.
+template <int NP>
+__device__ __forceinline__ void line_diff(const float v[NP][kNeq], float d[NP][kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        d[0][c] = 0.0f;
+#pragma unroll
+        for (int k = 1; k < NP; ++k) d[k][c] = v[k][c] - v[0][c];
+    }
+}
+
+// trace with weights summing to one
+template <int NP>
+__device__ __forceinline__ void trace_pu(const float* w, const float v[NP][kNeq],
+                                         const float d[NP][kNeq], float out[kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 1; k < NP; ++k) s += w[k] * d[k][c];
+        out[c] = v[0][c] + s;
+    }
+}
+
+// trace with weights summing to zero
+template <int NP>
+__device__ __forceinline__ void trace_zs(const float* w, const float d[NP][kNeq], float out[kNeq]) {
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 1; k < NP; ++k) s += w[k] * d[k][c];
+        out[c] = s;
+    }
+}
+
+// D v along the line (rows of D sum to zero)
+template <int NP>
+__device__ __forceinline__ void deriv_zs(const float* D, const float d[NP][kNeq], float out[NP][kNeq]) {
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+#pragma unroll
+        for (int c = 0; c < kNeq; ++c) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 1; k < NP; ++k) s += D[m * NP + k] * d[k][c];
+            out[m][c] = s;
+        }
+}
+
 // One line (ROW: row `line`, else column `line`) of a halo element reduced to its trace with
 // weights l, and for BR0 to the trace geps * (dl . v) of its viscous flux; cs = component stride.
 template <int NP, bool BR0, bool ROW>
@@ -66,12 +122,13 @@ __device__ __forceinline__ void halo_line(const double* __restrict__ u, int nel,
         load_row<NP>(u, nel, en, line, v);
     else
         load_col<NP>(u, nel, en, line, v);
-    float tr[kNeq];
-    trace<NP>(l, v, tr);
+    float d[NP][kNeq], tr[kNeq];
+    line_diff<NP>(v, d);
+    trace_pu<NP>(l, v, d, tr);
 #pragma unroll
     for (int c = 0; c < kNeq; ++c) su[c * cs] = tr[c];
     if constexpr (BR0) {
-        trace<NP>(dl, v, tr);
+        trace_zs<NP>(dl, d, tr);
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) sg[c * cs] = geps * tr[c];
     }
@@ -91,23 +148,26 @@ __device__ __forceinline__ void sweep(const OpsM<NP>& op, const float u[NP][kNeq
                                       float s, float V[NP][kNeq], float tHL[kNeq],
                                       float tHR[kNeq], float uL[kNeq], float uR[kNeq],
                                       float gL[kNeq], float gR[kNeq]) {
-    float H[NP][kNeq];
+    float H[NP][kNeq], d[NP][kNeq];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const Prim<float> q = primitives(u[k], gamma);
         axis_flux<DIR>(u[k], q, H[k]);
     }
-    trace<NP>(op.lL, u, uL);
-    trace<NP>(op.lR, u, uR);
+    line_diff<NP>(u, d);
+    trace_pu<NP>(op.lL, u, d, uL);
+    trace_pu<NP>(op.lR, u, d, uR);
     if constexpr (CAPT != 0) {
         float Du[NP][kNeq];
-        line_deriv<NP>(op.D, u, 1.0f, Du);
+        deriv_zs<NP>(op.D, d, Du);
         if constexpr (CAPT == 2) {   // corrections of the DD form use the traces of F
-            trace<NP>(op.lL, H, tHL);
-            trace<NP>(op.lR, H, tHR);
+            float dF[NP][kNeq];
+            line_diff<NP>(H, dF);
+            trace_pu<NP>(op.lL, H, dF, tHL);
+            trace_pu<NP>(op.lR, H, dF, tHR);
         } else {   // face values of D u as dL . u, exactly as the halo lines form them
-            trace<NP>(op.dL, u, gL);
-            trace<NP>(op.dR, u, gR);
+            trace_zs<NP>(op.dL, d, gL);
+            trace_zs<NP>(op.dR, d, gR);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
                 gL[c] *= s;
@@ -119,11 +179,13 @@ __device__ __forceinline__ void sweep(const OpsM<NP>& op, const float u[NP][kNeq
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) H[k][c] -= s * Du[k][c];
     }
+    float dH[NP][kNeq];
+    line_diff<NP>(H, dH);
     if constexpr (CAPT != 2) {
-        trace<NP>(op.lL, H, tHL);
-        trace<NP>(op.lR, H, tHR);
+        trace_pu<NP>(op.lL, H, dH, tHL);
+        trace_pu<NP>(op.lR, H, dH, tHR);
     }
-    line_deriv<NP>(op.D, H, 1.0f, V);
+    deriv_zs<NP>(op.D, dH, V);
 }
 
 // ---- Line contractions on the matrix cores (Np <= 4).
@@ -176,11 +238,13 @@ __device__ __forceinline__ void sweep_mf(const LineMats<NP>& A, const float u[NP
         const Prim<float> q = primitives(u[k], gamma);
         axis_flux<DIR>(u[k], q, H[k]);
     }
+    float d[NP][kNeq];
+    line_diff<NP>(u, d);
 #pragma unroll
     for (int c = 0; c < kNeq; ++c) {
-        const f4 tr = line_mv<NP>(A.T1, u, c);
-        uL[c] = tr[0];
-        uR[c] = tr[1];
+        const f4 tr = line_mv<NP>(A.T1, d, c);
+        uL[c] = u[0][c] + tr[0];
+        uR[c] = u[0][c] + tr[1];
         if constexpr (CAPT == 1) {
             gL[c] = s * tr[2];
             gR[c] = s * tr[3];
@@ -188,28 +252,32 @@ __device__ __forceinline__ void sweep_mf(const LineMats<NP>& A, const float u[NP
     }
     if constexpr (CAPT != 0) {
         if constexpr (CAPT == 2) {   // corrections of the DD form use the traces of F
+            float dF[NP][kNeq];
+            line_diff<NP>(H, dF);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
-                const f4 tr = line_mv<NP>(A.T2, H, c);
-                tHL[c] = tr[0];
-                tHR[c] = tr[1];
+                const f4 tr = line_mv<NP>(A.T2, dF, c);
+                tHL[c] = H[0][c] + tr[0];
+                tHR[c] = H[0][c] + tr[1];
             }
         }
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) {
-            const f4 du = line_mv<NP>(A.Dm, u, c);
+            const f4 du = line_mv<NP>(A.Dm, d, c);
 #pragma unroll
             for (int k = 0; k < NP; ++k) H[k][c] -= s * du[k];
         }
     }
+    float dH[NP][kNeq];
+    line_diff<NP>(H, dH);
 #pragma unroll
     for (int c = 0; c < kNeq; ++c) {
         if constexpr (CAPT != 2) {
-            const f4 tr = line_mv<NP>(A.T2, H, c);
-            tHL[c] = tr[0];
-            tHR[c] = tr[1];
+            const f4 tr = line_mv<NP>(A.T2, dH, c);
+            tHL[c] = H[0][c] + tr[0];
+            tHR[c] = H[0][c] + tr[1];
         }
-        const f4 v = line_mv<NP>(A.Dm, H, c);
+        const f4 v = line_mv<NP>(A.Dm, dH, c);
 #pragma unroll
         for (int m = 0; m < NP; ++m) V[m][c] = v[m];
     }

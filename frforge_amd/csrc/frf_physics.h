@@ -44,8 +44,12 @@ struct Prim {
     R p;     // pressure (no floor)
 };
 
+// primitives and axis_flux are evaluated unfused: the same flux at a solution point and at a
+// face trace with the same bits then gives the same bits whatever the surrounding code lets the
+// compiler fuse, so a uniform state has F^ - F = 0 exactly in every precision.
 template <typename R>
 __device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
+#pragma clang fp contract(off)
     Prim<R> q;
     q.rho = rmax(U[0], R(kEps64));
     q.inv = rcp_(q.rho);
@@ -170,6 +174,7 @@ __device__ __forceinline__ void iflux(const R uL[4], const R uR[4], R nx, R ny, 
 // F . n (Euler2D.jl:54-79) for n = e_DIR; q.p unfloored as in physical_flux_x/y
 template <int DIR, typename R>
 __device__ __forceinline__ void axis_flux(const R U[4], const Prim<R>& q, R out[4]) {
+#pragma clang fp contract(off)
     constexpr int N = 1 + DIR, T = 2 - DIR;
     const R un = DIR == 0 ? q.u : q.v;
     out[0] = U[N];

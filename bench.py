@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched 64-instance ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=16)
-    ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null")
+    ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null,"
+                                       "fp32_mixed:learned")
     ap.add_argument("--profile-only", default="", help="time only these form:capturing pairs (for rocprof)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy whose HBM bytes are known, to "
@@ -70,7 +71,11 @@ def build_state(nx, p, precision, capturing):
     st = allocate_state(mesh, ops, scheme=SchemeConfig(flux="rusanov"), precision=precision)
     cfg = seeded_quadrant_states(SEED)
     set_initial_condition(st, lambda x, y: riemann2d_ic(eq, x, y, config=cfg))
-    method = PerssonAVMethod.create(c_av=0.1) if capturing == "persson_br0" else NullCapturing()
+    if capturing == "learned":
+        from frforge_amd.learned import LearnedSensorMethod
+        method = LearnedSensorMethod.create(c_av=0.1)
+    else:
+        method = PerssonAVMethod.create(c_av=0.1) if capturing == "persson_br0" else NullCapturing()
     return eq, st, method, cfg
 
 
@@ -196,6 +201,35 @@ def run_cpu_baseline(nx, p, dt, capturing="persson_br0"):
                       f"({el:.1f} s, oracle C restatement, OpenMP {threads} threads)"}
 
 
+def learned_leg(ls, hs, l1, nx, p):
+    """The learned-sensor configuration: the same workload with the MLP sensor in the stage
+    epilogue (bf16 MFMA), its throughput beside the Persson form, the density difference after
+    the same steps, the sensor agreement on the initial state and the scoring harness's verdict."""
+    from frforge_amd.learned import (LearnedSensorMethod, element_modes, judge_against_persson,
+                                     persson_sigma_from_modes)
+    from frforge_amd.solver import sense
+    eq, st, method, _ = build_state(nx, p, "fp32_mixed", "learned")
+    s_l = np.zeros(st.mesh.n_elements)
+    sense(s_l, method, st.u, st, eq)
+    st.device.close()
+    st.device = None
+    s_p = persson_sigma_from_modes(element_modes(st.u, st.ops.V_legendre))
+    verdict = judge_against_persson()
+    out = {"dof_stage_per_s": ls["dof_stage_per_s"], "ms_per_step": ls["ms_per_step"],
+           "stage_kernel_us": ls["stage_kernel_us"],
+           "throughput_vs_persson_mixed": ls["dof_stage_per_s"] / hs["dof_stage_per_s"],
+           "density_l1_vs_persson_mixed": l1[0], "density_linf_vs_persson_mixed": l1[1],
+           "sigma_mae_vs_persson_initial_state": float(np.mean(np.abs(s_l - s_p))),
+           "sigma_max_abs_diff_initial_state": float(np.max(np.abs(s_l - s_p))),
+           "judge": verdict,
+           "model": "MLP 16-32-32-1 ReLU on the Legendre modes of density, bf16 weights and "
+                    "activations, FP32 accumulation (v_mfma_f32_16x16x16bf16), Persson fallback"}
+    print(f"# learned sensor: {ls['ms_per_step']:.3f} ms/step, sigma MAE vs Persson "
+          f"{out['sigma_mae_vs_persson_initial_state']:.3e}, judge selects {verdict['selected']} "
+          f"(composite delta {verdict['composite_delta']:+.4f})", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -259,6 +293,10 @@ def main():
         extra["null_fp32_mixed"] = summary(("fp32_mixed", "null"))
         extra["null_fp64"] = summary(("fp64", "null"))
         extra["null_density_l1_mixed_vs_fp64"], _ = l1_density(("fp32_mixed", "null"), ("fp64", "null"))
+
+    if ("fp32_mixed", "learned") in res:
+        extra["learned_sensor"] = learned_leg(summary(("fp32_mixed", "learned")), hs,
+                                              l1_density(("fp32_mixed", "learned"), head), nx, p)
 
     ens = None
     if not a.no_ensemble:

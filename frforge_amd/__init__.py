@@ -7,6 +7,7 @@ include/frforge_hip.h (libfrforge_hip.so).
 from .capturing import (AbstractCapturingMethod, ElementArtificialViscosity, NullCapturing,
                         PerssonAVMethod, PerssonSensor, ScaledPerssonMethod, get_capturing_method,
                         list_methods, method_params, register_method)
+from .learned import LearnedSensorMethod   # registers "learned_persson"
 from .equations import (Euler2D, conserved_to_primitives, pressure, primitives_to_conserved,
                         sound_speed)
 from .mesh import Mesh2D, PeriodicBC, TransmissiveBC, build_mesh_metrics

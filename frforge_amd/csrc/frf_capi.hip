@@ -16,6 +16,7 @@
 #include "../../include/frforge_hip.h"
 #include "frf_kernels.h"
 #include "frf_launch.h"
+#include "frf_learned.h"
 
 using frf::DevCtl;
 using frf::HostOps;
@@ -36,6 +37,7 @@ struct frf_ctx {
     float* dRtabF = nullptr;
     DevCtl* dCtl = nullptr;
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
+    frf::MlpPacked* dMlp = nullptr;    // learned sensor weights, when set
     HostOps ops{};
     bool ops_set = false, geom_set = false, state_set = false;
     std::vector<double> xi_nodes;     // solution points, for the metric tables
@@ -144,6 +146,7 @@ extern "C" int32_t frf_destroy(frf_ctx* c) {
     if (c->dCtabF) (void)hipFree(c->dCtabF);
     if (c->dRtabF) (void)hipFree(c->dRtabF);
     if (c->dCtl) (void)hipFree(c->dCtl);
+    if (c->dMlp) (void)hipFree(c->dMlp);
     if (c->hCtl) (void)hipHostFree(c->hCtl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -219,6 +222,31 @@ extern "C" int32_t frf_set_operators(frf_ctx* c, const double* xi, const double*
 // and their Lagrange extrapolations to the faces (:124-184).  Computed here in the reference's
 // summation order without fused multiply-adds, so the device reproduces the reference's
 // metric values and normals exactly.
+extern "C" int32_t frf_set_learned_sensor(frf_ctx* c, const void* packed, int64_t nbytes) {
+    if (!c) return FRF_E_ARG;
+    if (!packed) {   // back to the Persson sigma
+        if (c->dMlp) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->dMlp));
+            c->dMlp = nullptr;
+        }
+        return FRF_OK;
+    }
+    if (nbytes != (int64_t)sizeof(frf::MlpPacked))
+        return fail(c, FRF_E_ARG, "learned sensor weights: %lld bytes, expected %zu", (long long)nbytes,
+                    sizeof(frf::MlpPacked));
+    if (c->pb.p != 3)
+        return fail(c, FRF_E_UNSUPPORTED, "the learned sensor takes the 16 modes of p = 3 (got p = %d)", c->pb.p);
+    if (c->pb.precision != FRF_FP32_MIXED)
+        return fail(c, FRF_E_UNSUPPORTED, "the learned sensor runs in the FP32-mixed form");
+    if (c->pb.capturing != FRF_CAPT_PERSSON_BR0 && c->pb.capturing != FRF_CAPT_PERSSON_DD)
+        return fail(c, FRF_E_UNSUPPORTED, "the learned sensor replaces the Persson sigma of a Persson AV method");
+    if (!c->dMlp) HIPCHK(c, hipMalloc(&c->dMlp, sizeof(frf::MlpPacked)));
+    HIPCHK(c, hipMemcpyAsync(c->dMlp, packed, sizeof(frf::MlpPacked), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FRF_OK;
+}
+
 extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* yv) {
 #pragma clang fp contract(off)
     if (!c) return FRF_E_ARG;
@@ -385,6 +413,7 @@ static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double
     P.stage = stage;
     P.want_lambda = want_lambda;
     P.p = c->pb.p;
+    P.mlp = c->dMlp;
     return P;
 }
 
@@ -406,6 +435,7 @@ static frf::ElemParams elem_params(frf_ctx* c, const double* u, int want_sigma, 
     E.nel = c->nel;
     E.p = c->pb.p;
     E.want_sigma = want_sigma;
+    E.mlp = c->dMlp;
     return E;
 }
 

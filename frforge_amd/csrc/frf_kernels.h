@@ -74,6 +74,8 @@ static_assert(TabLayout<2>::STRIDE == tab_stride(2) && TabLayout<3>::STRIDE == t
                   TabLayout<6>::STRIDE == tab_stride(6),
               "host and device metric-table layouts differ");
 
+struct MlpPacked;         // learned sensor weights (frf_learned.h)
+
 struct StageParams {
     const double* uin;    // state the residual is evaluated on
     const double* u0;     // u^n (stages 2 and 3)
@@ -94,6 +96,7 @@ struct StageParams {
     int nblk;
     int p;
     int resident, ncu;    // blocks of the stage kernel resident per CU, and CUs (start staggering)
+    const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3, mixed form), or null
 };
 
 struct ElemParams {
@@ -107,6 +110,7 @@ struct ElemParams {
     int nx, ny, nel, nblk;
     int p;
     int want_sigma;       // compute sigma/eps (capturing) in addition to the wave speed
+    const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3), or null
 };
 
 }  // namespace frf

@@ -9,6 +9,7 @@
 #include "frf_device.h"
 #include "frf_kernels.h"
 #include "frf_launch.h"
+#include "frf_learned.h"
 #include "frf_physics.h"
 
 namespace frf {
@@ -23,6 +24,10 @@ __global__ void __launch_bounds__(EPB* NP) elem_kernel(ElemParams P, OpsT<NP, do
     __shared__ R sT[EPB][NP][NP];
     __shared__ R sE[EPB][NP][2];
     __shared__ double sL[EPB * NP];
+    // learned sensor (p = 3): modal coefficients and network output per element, MLP scratch
+    constexpr bool LEARN = NP == 4 && EPB % 16 == 0;
+    __shared__ float sM[LEARN ? EPB : 1][17];
+    __shared__ short sSc[LEARN ? EPB / 16 : 1][512];
     if (check_ctl && (P.ctl->done != 0 || P.ctl->fail_stage < kNoFail)) return;
     const int tid = threadIdx.x;
     const int le = tid / NP;
@@ -63,12 +68,24 @@ __global__ void __launch_bounds__(EPB* NP) elem_kernel(ElemParams P, OpsT<NP, do
             et += s * s;
             if (a == NP - 1) eh += s * s;
             else ecol += s * s;
+            if constexpr (LEARN) sM[le][a + NP * b] = float(s);
         }
         if (b == NP - 1) eh += ecol;
         sE[le][b][0] = et;
         sE[le][b][1] = eh;
     }
     __syncthreads();
+    if constexpr (LEARN) {
+        if (P.want_sigma && P.mlp) {   // wave w evaluates elements 16 w .. 16 w + 15
+            const int w = tid >> 6, lane = tid & 63;
+            const mlp_f4 o = mlp_wave16(P.mlp, &sM[16 * w + (lane & 15)][0], float(P.eps_floor), sSc[w]);
+            if ((lane & 15) == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sM[16 * w + 4 * (lane >> 4) + r][16] = o[r];
+            }
+            __syncthreads();
+        }
+    }
     if (valid && t == 0) {
         double lm = 0.0;
 #pragma unroll
@@ -92,6 +109,9 @@ __global__ void __launch_bounds__(EPB* NP) elem_kernel(ElemParams P, OpsT<NP, do
             } else {
                 se = log10f(ehigh / etot + float(P.eps_floor));
                 se = 1.0f / (1.0f + expf(-float(P.kappa) * (se - float(P.s0))));
+            }
+            if constexpr (LEARN) {
+                if (P.mlp) se = R(mlp_sigma_or(sM[le][16], float(se)));
             }
             const int jx = e % P.nx, jy = e / P.nx;
             const double dx = P.ctab[(size_t)jx * CT + L::DELTA];

@@ -37,6 +37,7 @@
 #include "frf_device.h"
 #include "frf_kernels.h"
 #include "frf_launch.h"
+#include "frf_learned.h"
 #include "frf_physics.h"
 
 #ifndef FRF_STAGGER
@@ -324,6 +325,11 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
     constexpr int FS = NT * NP;                 // component stride of the per-element face arrays
     constexpr int HX = TY * NP, HY = TX * NP;   // ... of the tile-edge (W/E, S/N) arrays
     constexpr int NW = (NTHR + 63) / 64;
+    // learned sensor (p = 3): 16 elements per wave, modes and output in the element's slot
+    constexpr bool LEARN = AV && NP == 4 && NT == 16 * NW;
+    constexpr int kModes = NPP + 3 * NP;
+    static_assert(!LEARN || kModes + NPP + 1 <= ESTR, "modes fit an element slot");
+    static_assert(!LEARN || kNeq * FS * 4 >= NW * 1024, "MLP scratch fits the face arrays");
     // contractions on the matrix cores for Np <= 4 (one 4-row group per matrix)
     constexpr bool MF = FRF_MFMA && NP <= 4 && NTHR % 64 == 0;
 
@@ -755,12 +761,25 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                     eh += s * s;
                 else
                     ecol += s * s;
+                if constexpr (LEARN) sT[kModes + a + NP * b] = s;
             }
             if (b == NP - 1) eh += ecol;
             sT[NPP + NP + t] = et;
             sT[NPP + 2 * NP + t] = eh;
         }
         __syncthreads();
+        if constexpr (LEARN) {
+            if (P.mlp) {   // learned sensor: wave w evaluates elements 16 w .. 16 w + 15
+                const int w = tid >> 6, lane = tid & 63;
+                const mlp_f4 o = mlp_wave16(P.mlp, sU + (16 * w + (lane & 15)) * ESTR + kModes,
+                                            float(P.eps_floor), reinterpret_cast<short*>(sXW) + w * 512);
+                if ((lane & 15) == 0) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sU[(16 * w + 4 * (lane >> 4) + r) * ESTR + kModes + NPP] = o[r];
+                }
+                __syncthreads();
+            }
+        }
     }
     float lm_el = 0.0f;
     if (valid && t == 0) {
@@ -776,7 +795,10 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                 ehigh += sT[NPP + 2 * NP + k];
             }
             etot += float(P.eps_floor);
-            const float sigma = persson_sigma_hw(ehigh, etot, float(P.s0), float(P.kappa), float(P.eps_floor));
+            float sigma = persson_sigma_hw(ehigh, etot, float(P.s0), float(P.kappa), float(P.eps_floor));
+            if constexpr (LEARN) {
+                if (P.mlp) sigma = mlp_sigma_or(sT[kModes + NPP], sigma);
+            }
             const float h = dx < dy ? dx : dy;
             const int pp = P.p > 1 ? P.p : 1;
             P.eps_out[e] = sigma > 0.0f ? double(float(P.c_av) * sigma * (h / float(pp)) * lm) : 0.0;

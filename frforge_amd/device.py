@@ -22,7 +22,8 @@ CAPTURING = {"null": 0, "persson_br0": 1, "persson_dd": 2}
 STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
-EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_upload_state",
+EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_learned_sensor",
+            "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
             "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy",
@@ -82,6 +83,7 @@ def load_library(path: str = LIB_PATH):
     L.frf_create.argtypes = [C.POINTER(FrfProblem), C.POINTER(vp)]
     L.frf_set_operators.argtypes = [vp] + [dp] * 8
     L.frf_set_geometry.argtypes = [vp, dp, dp]
+    L.frf_set_learned_sensor.argtypes = [vp, C.c_void_p, C.c_int64]
     L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf_residual.argtypes = [vp, dp, C.c_int64]
@@ -197,6 +199,15 @@ class DeviceContext:
         xv = np.ascontiguousarray(x_vertices, dtype=np.float64)
         yv = np.ascontiguousarray(y_vertices, dtype=np.float64)
         self._chk(self.lib.frf_set_geometry(self.h, _dp(xv), _dp(yv)), "frf_set_geometry")
+
+    def set_learned_sensor(self, packed):
+        """Weights of the learned sensor (bytes from MLPWeights.pack), or None for Persson."""
+        if packed is None:
+            self._chk(self.lib.frf_set_learned_sensor(self.h, None, 0), "frf_set_learned_sensor")
+            return
+        buf = C.create_string_buffer(bytes(packed), len(packed))
+        self._chk(self.lib.frf_set_learned_sensor(self.h, C.cast(buf, C.c_void_p), len(packed)),
+                  "frf_set_learned_sensor")
 
     def _check_state(self, u):
         if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.f_contiguous

@@ -1,0 +1,357 @@
+"""Learned troubled-cell sensor for p = 3: a small MLP that maps the 16 Legendre modal
+coefficients of an element's density to the Persson indicator sigma, trained on the solver's
+own (modes, sigma) pairs and run on the device inside the stage kernels' epilogue.
+
+It plugs into the reference's capturing interface at `sense!` (Interfaces.jl:79-87): the
+element viscosity keeps the unchanged formula eps = c_av sigma h / max(p,1) lambda_e
+(PerssonAV2D.jl:89-109), BR0 dissipation is unchanged, and the Persson sigma is the fallback
+for any element whose network output is not finite.  The training data are what the
+reference's per-element diagnostics dump (compute_capturing_diagnostics_2d,
+VTKHighOrder.jl:537-559) gives: the Persson sigma of each element of FP64 device runs.
+
+Features: for modal coefficients m_k (k = a + 4 b, a the xi mode and b the eta mode) and
+E = sum m_k^2 + eps_floor, x_k = clip(log10(m_k^2 / E + 1e-10), -10, 0) / 10 + 1, in [0, 1].
+The floor 1e-10 sits far above the single-precision rounding of the modal transform on the
+device (|m_k| / sqrt(E) ~ 1e-7, energy fraction ~ 1e-14), so a mode that is zero in exact
+arithmetic gives x_k = 0 on the device as on the host, and far below the energy fractions
+(> 1e-4) at which the Persson sigma departs from zero.
+Network: h1 = relu(W1 x + b1) (32), h2 = relu(W2 h1 + b2) (32), sigma = clip(W3 h2 + b3, 0, 1).
+Inference: x, h1, h2 and the weights rounded to bfloat16 (nearest even), products accumulated in
+FP32, biases added in FP32 -- what the device's v_mfma_f32_16x16x16bf16 path computes.
+Training is FP32 (Adam on a weighted squared error).
+"""
+from __future__ import annotations
+
+import io
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from .capturing import (METHOD_REGISTRY, AbstractCapturingMethod, ElementArtificialViscosity,
+                        PerssonAVMethod, PerssonSensor)
+
+NP = 4
+NMODES = NP * NP
+HIDDEN = 32
+FEATURE_FLOOR = 1e-10
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "learned_sensor_p3.npz")
+
+
+# ---------------------------------------------------------------------------- features
+
+def element_modes(u: np.ndarray, V_legendre: np.ndarray) -> np.ndarray:
+    """(Nel, 16) modal coefficients V^{-1} rho V^{-T} of the density of a 2D state u[i,j,e,c]
+    (PerssonAV2D.jl:40-46), flattened column-major: k = a + Np b."""
+    Vinv = np.linalg.inv(np.asarray(V_legendre, dtype=np.float64))
+    rho = np.asarray(u[:, :, :, 0], dtype=np.float64)          # (i, j, e)
+    m = np.einsum("ai,ije,bj->abe", Vinv, rho, Vinv)           # (a, b, e)
+    return np.ascontiguousarray(m.reshape(NMODES, -1, order="F").T)
+
+
+def features(modes: np.ndarray, eps_floor: float = 1e-16) -> np.ndarray:
+    m2 = np.asarray(modes, dtype=np.float64) ** 2
+    etot = m2.sum(axis=1, keepdims=True) + eps_floor
+    x = np.log10(m2 / etot + FEATURE_FLOOR)
+    return (np.clip(x, -10.0, 0.0) / 10.0 + 1.0).astype(np.float32)
+
+
+def persson_sigma_from_modes(modes: np.ndarray, p: int = 3, kappa: float = 4.0, s0_factor: float = -4.0,
+                             eps_floor: float = 1e-16) -> np.ndarray:
+    """The Persson sigma of the same modes (PerssonAV2D.jl:47-56, SharedAV.jl:9-21): energy of
+    the highest mode in either direction over the total, logistic in log10."""
+    m = np.asarray(modes, dtype=np.float64).reshape(-1, NP, NP, order="C")   # [e, b, a]
+    m2 = m ** 2
+    etot = m2.sum(axis=(1, 2)) + eps_floor
+    ehigh = m2[:, :, NP - 1].sum(axis=1) + m2[:, NP - 1, :NP - 1].sum(axis=1)
+    s0 = s0_factor * math.log10(max(p, 1))
+    s = np.log10(ehigh / etot + eps_floor)
+    return 1.0 / (1.0 + np.exp(-kappa * (s - s0)))
+
+
+# ---------------------------------------------------------------------------- bf16 inference
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """Round float32 values to the nearest bfloat16 (ties to even), returned as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def _bf16_bits(x: np.ndarray) -> np.ndarray:
+    return (bf16_round(x).view(np.uint32) >> 16).astype(np.uint16)
+
+
+@dataclass
+class MLPWeights:
+    W1: np.ndarray   # (32, 16)
+    b1: np.ndarray   # (32,)
+    W2: np.ndarray   # (32, 32)
+    b2: np.ndarray   # (32,)
+    W3: np.ndarray   # (1, 32)
+    b3: np.ndarray   # (1,)
+    meta: Dict[str, float] = field(default_factory=dict)
+
+    @staticmethod
+    def init(seed: int) -> "MLPWeights":
+        rng = np.random.default_rng(seed)
+        he = lambda n_out, n_in: (rng.standard_normal((n_out, n_in)) * math.sqrt(2.0 / n_in)).astype(np.float32)
+        return MLPWeights(he(HIDDEN, NMODES), np.zeros(HIDDEN, np.float32), he(HIDDEN, HIDDEN),
+                          np.zeros(HIDDEN, np.float32), he(1, HIDDEN) * 0.1, np.zeros(1, np.float32))
+
+    def forward_fp32(self, x: np.ndarray):
+        z1 = x @ self.W1.T + self.b1
+        h1 = np.maximum(z1, 0.0)
+        z2 = h1 @ self.W2.T + self.b2
+        h2 = np.maximum(z2, 0.0)
+        out = h2 @ self.W3.T + self.b3
+        return z1, h1, z2, h2, out[:, 0]
+
+    def predict_bf16(self, x: np.ndarray) -> np.ndarray:
+        """sigma as the device computes it (bf16 operands, FP32 accumulation)."""
+        xb = bf16_round(np.asarray(x, np.float32))
+        h1 = np.maximum(xb.astype(np.float64) @ bf16_round(self.W1).T.astype(np.float64) + self.b1, 0.0)
+        h1 = bf16_round(h1.astype(np.float32))
+        h2 = np.maximum(h1.astype(np.float64) @ bf16_round(self.W2).T.astype(np.float64) + self.b2, 0.0)
+        h2 = bf16_round(h2.astype(np.float32))
+        out = h2.astype(np.float64) @ bf16_round(self.W3).T.astype(np.float64) + self.b3
+        return np.clip(out[:, 0], 0.0, 1.0).astype(np.float32)
+
+    # -------------------------------------------------------------- device packing
+    def pack(self) -> bytes:
+        """The layout the device reads (frf_learned.h, MlpPacked): for every 16x16x16 MFMA the B
+        operand in lane order -- lane l holds B[k = 4 (l / 16) + r][n = l % 16], r = 0..3 -- as
+        bfloat16, then the FP32 biases.  B[k][n] = W[n0 + n][k0 + k] (B = W^T)."""
+        def block(W, n0, k0, nmax):
+            out = np.zeros((64, 4), np.float32)
+            for lane in range(64):
+                for r in range(4):
+                    k, n = 4 * (lane // 16) + r, lane % 16
+                    if n0 + n < nmax:
+                        out[lane, r] = W[n0 + n, k0 + k]
+            return _bf16_bits(out)
+        parts = [block(self.W1, 0, 0, HIDDEN), block(self.W1, 16, 0, HIDDEN)]
+        for kb in range(2):
+            for nb in range(2):
+                parts.append(block(self.W2, 16 * nb, 16 * kb, HIDDEN))
+        for kb in range(2):
+            parts.append(block(self.W3, 0, 16 * kb, 1))
+        b = np.concatenate([self.b1, self.b2, self.b3, np.zeros(3, np.float32)]).astype(np.float32)
+        return b"".join(p.tobytes() for p in parts) + b.tobytes()
+
+    def save(self, path: str) -> None:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez(path, W1=self.W1, b1=self.b1, W2=self.W2, b2=self.b2, W3=self.W3, b3=self.b3,
+                 meta=np.array(json_dumps(self.meta)))
+
+    @staticmethod
+    def load(path: str = DEFAULT_WEIGHTS) -> "MLPWeights":
+        with np.load(path, allow_pickle=False) as z:
+            meta = json_loads(str(z["meta"])) if "meta" in z else {}
+            return MLPWeights(z["W1"].astype(np.float32), z["b1"].astype(np.float32),
+                              z["W2"].astype(np.float32), z["b2"].astype(np.float32),
+                              z["W3"].astype(np.float32), z["b3"].astype(np.float32), meta)
+
+
+PACKED_BYTES = 8 * 64 * 4 * 2 + (HIDDEN + HIDDEN + 4) * 4
+
+
+def json_dumps(d) -> str:
+    import json
+    return json.dumps(d, sort_keys=True)
+
+
+def json_loads(s: str):
+    import json
+    return json.loads(s)
+
+
+# ---------------------------------------------------------------------------- training
+
+def train(x: np.ndarray, y: np.ndarray, seed: int = 7, iters: int = 8000, batch: int = 4096,
+          lr: float = 3e-3, weight_hot: float = 4.0, weight_under: float = 3.0) -> MLPWeights:
+    """Adam on sum w (sigma_hat - sigma)^2 / sum w in FP32; samples with sigma > 0.01 weigh
+    `weight_hot` times more (they are the minority and the ones that set the viscosity), and an
+    under-prediction weighs `weight_under` times more than an over-prediction (too little
+    viscosity at a shock costs an overshoot, too much only some extra dissipation)."""
+    rng = np.random.default_rng(seed)
+    net = MLPWeights.init(seed)
+    params = ["W1", "b1", "W2", "b2", "W3", "b3"]
+    m = {k: np.zeros_like(getattr(net, k)) for k in params}
+    v = {k: np.zeros_like(getattr(net, k)) for k in params}
+    b1c, b2c, eps = 0.9, 0.999, 1e-8
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    w_all = np.where(y > 0.01, weight_hot, 1.0).astype(np.float32)
+    n = len(y)
+    for it in range(1, iters + 1):
+        idx = rng.integers(0, n, size=min(batch, n))
+        xb, yb, wb = x[idx], y[idx], w_all[idx]
+        z1, h1, z2, h2, out = net.forward_fp32(xb)
+        wb = wb * np.where(out < yb, weight_under, 1.0).astype(np.float32)
+        g_out = (2.0 * wb * (out - yb) / wb.sum())[:, None].astype(np.float32)
+        grads = {"W3": g_out.T @ h2, "b3": g_out.sum(0)}
+        g_h2 = (g_out @ net.W3) * (z2 > 0)
+        grads["W2"], grads["b2"] = g_h2.T @ h1, g_h2.sum(0)
+        g_h1 = (g_h2 @ net.W2) * (z1 > 0)
+        grads["W1"], grads["b1"] = g_h1.T @ xb, g_h1.sum(0)
+        a = lr * (0.1 if it > 0.75 * iters else 1.0)
+        for k in params:
+            g = grads[k].astype(np.float32)
+            m[k] = b1c * m[k] + (1 - b1c) * g
+            v[k] = b2c * v[k] + (1 - b2c) * g * g
+            mh = m[k] / (1 - b1c ** it)
+            vh = v[k] / (1 - b2c ** it)
+            setattr(net, k, (getattr(net, k) - a * mh / (np.sqrt(vh) + eps)).astype(np.float32))
+    return net
+
+
+def collect_pairs(snapshots: List[Tuple[np.ndarray, np.ndarray, np.ndarray]], keep_cold: float = 0.05,
+                  seed: int = 11) -> Tuple[np.ndarray, np.ndarray]:
+    """(features, sigma) from (u, V_legendre, sigma) snapshots: every element with sigma > 1e-3
+    and a `keep_cold` sample of the rest."""
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    for u, V, sigma in snapshots:
+        f = features(element_modes(u, V))
+        hot = sigma > 1e-3
+        keep = hot | (rng.random(len(sigma)) < keep_cold)
+        xs.append(f[keep])
+        ys.append(sigma[keep].astype(np.float32))
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def generate_snapshots(riemann_seeds=(1, 3, 5, 7, 11, 13), n: int = 192,
+                       times=(0.005, 0.01, 0.02, 0.035, 0.05),
+                       lax_liu=("cfg6", "cfg3"), lax_liu_n: int = 128, lax_liu_times=(0.02, 0.05, 0.08),
+                       vortex_n: int = 48, vortex_times=(0.25, 1.0)):
+    """FP64 device runs of configs 2 and 3 at reduced size (p = 3, Persson + BR0 c_av 0.1):
+    seeded four-quadrant Riemann problems and the Lax-Liu configurations 6 and 3 on [0,1]^2, and
+    the isentropic vortex, with the Persson sigma of every element at the given times (the
+    per-element diagnostics of the reference)."""
+    from .cases import riemann2d_ic, seeded_quadrant_states, vortex_ic
+    from .equations import Euler2D
+    from .mesh import Mesh2D, TransmissiveBC
+    from .operators import build_operators
+    from .solver import integrate, sense
+    from .state import allocate_state, set_initial_condition
+    eq = Euler2D(1.4)
+    ops = build_operators(3)
+    m = PerssonAVMethod.create(c_av=0.1)
+    out = []
+    T = TransmissiveBC
+    runs = [(seeded_quadrant_states(s), n, times) for s in riemann_seeds]
+    runs += [(name, lax_liu_n, lax_liu_times) for name in lax_liu]
+    for cfg, nn, tt in runs:
+        st = allocate_state(Mesh2D(0.0, 1.0, 0.0, 1.0, nn, nn, left_bc=T(), right_bc=T(), bottom_bc=T(),
+                                   top_bc=T()), ops)
+        set_initial_condition(st, lambda x, y: riemann2d_ic(eq, x, y, config=cfg))
+        for t in tt:
+            r = integrate(st, eq, m, t, cfl=0.08)
+            if r["status"] != "ok":
+                break
+            sig = np.zeros(st.mesh.n_elements)
+            sense(sig, m, st.u, st, eq)
+            out.append((st.u.copy(order="F"), ops.V_legendre, sig))
+        st.device.close()
+        st.device = None
+    st = allocate_state(Mesh2D(-5.0, 5.0, -5.0, 5.0, vortex_n, vortex_n), ops)
+    set_initial_condition(st, vortex_ic(eq, x0=0.0, y0=0.0))
+    for t in vortex_times:
+        integrate(st, eq, m, t, cfl=0.08)
+        sig = np.zeros(st.mesh.n_elements)
+        sense(sig, m, st.u, st, eq)
+        out.append((st.u.copy(order="F"), ops.V_legendre, sig))
+    st.device.close()
+    st.device = None
+    return out
+
+
+def fit_from_device(path: str = DEFAULT_WEIGHTS, seed: int = 7) -> Dict[str, float]:
+    """Generate the training snapshots on the device, train, save the weights; returns the fit
+    statistics (also stored in the weights' metadata)."""
+    snaps = generate_snapshots()
+    x, y = collect_pairs(snaps)
+    n = len(y)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    n_val = max(n // 10, 1)
+    val, tr = perm[:n_val], perm[n_val:]
+    net = train(x[tr], y[tr], seed=seed)
+    pv = net.predict_bf16(x[val])
+    err = np.abs(pv - y[val])
+    stats = {"samples": int(n), "train": int(len(tr)), "validation": int(n_val),
+             "val_mae": float(err.mean()), "val_p99_abs_err": float(np.quantile(err, 0.99)),
+             "val_max_abs_err": float(err.max()), "seed": seed}
+    net.meta = stats
+    net.save(path)
+    return stats
+
+
+# ---------------------------------------------------------------------------- method
+
+@dataclass(frozen=True)
+class LearnedSensorMethod(AbstractCapturingMethod):
+    """Persson AV with sigma from the learned sensor (p = 3, FP32-mixed device form); the
+    Persson sigma stands in for any element whose network output is not finite."""
+    inner: PerssonAVMethod = PerssonAVMethod.create()
+    weights_path: str = DEFAULT_WEIGHTS
+
+    @classmethod
+    def create(cls, kappa=4.0, s0_factor=-4.0, c_av=0.1, av_form="conservative_br0",
+               weights_path: Optional[str] = None):
+        return cls(PerssonAVMethod.create(kappa, s0_factor, c_av, av_form), weights_path or DEFAULT_WEIGHTS)
+
+    def device_capturing(self):
+        return self.inner.device_capturing()
+
+    def weights(self) -> MLPWeights:
+        return MLPWeights.load(self.weights_path)
+
+    def method_params(self):
+        p = self.inner.method_params()
+        p["method"] = "learned_persson"
+        p["sensor"] = "MLP 16-32-32-1 (bf16), Persson fallback"
+        return p
+
+
+METHOD_REGISTRY.setdefault("learned_persson", lambda **kw: LearnedSensorMethod.create(**kw))
+
+
+def judge_against_persson(delta: float = 0.02, nx: int = 64, t_final: float = 0.08,
+                          config="cfg6") -> Dict[str, object]:
+    """The scoring harness as the judge of the learned sensor: the four-quadrant Riemann case at
+    p = 3 (Lax-Liu configuration 6, transmissive, CFL 0.08; run_euler2d_riemann) with the Persson
+    sensor and with the learned one, both in the FP32-mixed form, scored with the absolute maps
+    (Scoring.jl:16-73).  The learned sensor is kept when its composite is within `delta` of
+    Persson's and its run passes; otherwise Persson AV is the fallback method."""
+    from .scoring import score_suite_absolute
+    from .suite2d import run_euler2d_riemann
+    out: Dict[str, object] = {}
+    cases = {}
+    for name, meth in (("persson_av", PerssonAVMethod.create(c_av=0.1)),
+                       ("learned_persson", LearnedSensorMethod.create())):
+        case, st, _ = run_euler2d_riemann(p=3, nx=nx, ny=nx, t_final=t_final, cfl=0.08, config=config,
+                                          method=meth, method_name=name, precision="fp32_mixed")
+        if st.device is not None:
+            st.device.close()
+            st.device = None
+        cases[name] = case
+        out[name] = {"pass": case["pass"], "overshoot": case["overshoot"],
+                     "composite": score_suite_absolute([case])["composite"]}
+    dl = out["learned_persson"]["composite"] - out["persson_av"]["composite"]
+    out["composite_delta"] = dl
+    out["selected"] = ("learned_persson" if cases["learned_persson"]["pass"] and dl >= -delta
+                       else "persson_av")
+    out["case"] = f"{config} p=3 {nx}x{nx} t={t_final}"
+    return out
+
+
+def select_sensor_method(delta: float = 0.02) -> AbstractCapturingMethod:
+    """The method the judge selects: the learned sensor, or Persson AV as the fallback."""
+    verdict = judge_against_persson(delta)
+    return LearnedSensorMethod.create() if verdict["selected"] == "learned_persson" else PerssonAVMethod.create()

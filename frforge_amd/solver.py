@@ -46,7 +46,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
     key = (state.p, mesh.nx, mesh.ny, tuple(mesh.x_vertices), tuple(mesh.y_vertices),
            _bc_name(mesh.left_bc), _bc_name(mesh.right_bc), _bc_name(mesh.bottom_bc),
            _bc_name(mesh.top_bc), state.scheme.flux, state.precision, kind,
-           tuple(sorted(prm.items())), float(eq.gamma), id(state.ops))
+           tuple(sorted(prm.items())), float(eq.gamma), id(state.ops),
+           getattr(method, "weights_path", None))
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
         return dev
@@ -59,6 +60,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
     ops = state.ops
     dev.set_operators(ops.xi, ops.w, ops.D, ops.ell_L, ops.ell_R, ops.gL_xi, ops.gR_xi, ops.V_legendre)
     dev.set_geometry(mesh.x_vertices, mesh.y_vertices)
+    if getattr(method, "weights_path", None) is not None:   # learned sensor (frforge_amd.learned)
+        dev.set_learned_sensor(method.weights().pack())
     dev.key = key
     state.device = dev
     return dev

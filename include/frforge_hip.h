@@ -77,6 +77,15 @@ int32_t frf_set_operators(frf_ctx* ctx, const double* xi, const double* w, const
  * rebuilds the affine metric terms of build_mesh_metrics (MeshMetrics2D.jl:82-218) from them. */
 int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y_vertices);
 
+/* Learned troubled-cell sensor in place of the Persson sigma, for the sense! hook of a Persson
+ * AV method (Interfaces.jl:79-87, PerssonAV2D.jl:20-59; the viscosity formula of
+ * PerssonAV2D.jl:89-109 is unchanged): an MLP 16 -> 32 -> 32 -> 1 on the 16 Legendre modes of
+ * the element density, bf16 on the matrix cores with FP32 accumulation; the Persson sigma stands
+ * in where the network output is not finite.  p = 3, FP32-mixed form, Persson BR0 or DD only.
+ * `packed`: the weights as frforge_amd.learned.MLPWeights.pack writes them (4368 bytes); NULL
+ * goes back to the Persson sigma.  No counterpart in the reference, which has no learned sensor. */
+int32_t frf_set_learned_sensor(frf_ctx* ctx, const void* packed, int64_t nbytes);
+
 /* Copy state.u (len = Np*Np*Nel*4 values) and state.t to / from the device. */
 int32_t frf_upload_state(frf_ctx* ctx, const double* u, int64_t len, double t);
 int32_t frf_download_state(frf_ctx* ctx, double* u, int64_t len, double* t);

@@ -1,0 +1,141 @@
+"""Learned sensor on the GPU (bf16 MFMA inference in the element pass and in the stage-kernel
+epilogue) against the host's bf16 reference of the same network and the oracle.
+
+Bars: device sigma equals the host bf16 inference on the same modes up to bf16 rounding of
+features that FP32 and FP64 modes place on different sides of a rounding boundary (>= 99 % of
+elements within 2e-3, all within 0.1); a non-finite network output gives the Persson sigma bit
+for bit; the residual and one SSP-RK3 step with the learned sensor equal the oracle's residual
+with the viscosity of the host network's sigma within 1e-3 (residual, normwise) and 2e-3 (the
+step's increment, on a discontinuous initial state, where the network's steepness turns the
+FP32-versus-FP64 differences of the intermediate states' modes into sigma differences); with a
+network of constant output the step equals the oracle's to the mixed form's own error (1e-5),
+which pins the epilogue's plumbing of the network output into the next stage's viscosity."""
+import numpy as np
+import pytest
+
+from frforge_amd.learned import (MLPWeights, LearnedSensorMethod, element_modes, features,
+                                 persson_sigma_from_modes)
+from frforge_amd.solver import residual, sense, ssp_rk3, ssp_rk3_step
+from oracle.oracle import Oracle
+
+from helpers import EQ, make_case, oracle_ops, rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+
+def _learned_case(nx=37, ny=21, seed=5, noise=0.01, weights=None):
+    st, _, _ = make_case(3, nx, ny, bc="transmissive", capturing="persson_br0", ic="riemann", seed=seed,
+                         noise=noise, precision="fp32_mixed")
+    m = LearnedSensorMethod.create(weights_path=weights)
+    return st, m
+
+
+def _host_sigma(u, st, w):
+    return w.predict_bf16(features(element_modes(u, st.ops.V_legendre))).astype(np.float64)
+
+
+def _null_oracle(st):
+    m = st.mesh
+    return Oracle(3, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(st.ops), bc_x="transmissive",
+                  bc_y="transmissive", capturing="null", c_av=0.1)
+
+
+def _oracle_residual_learned(orc, u, st, w):
+    eps = orc.element_viscosities(u, _host_sigma(u, st, w))
+    return orc.apply_br0(u, eps, orc.residual(u))
+
+
+def test_device_sigma_matches_host_inference():
+    st, m = _learned_case()
+    s = np.zeros(st.mesh.n_elements)
+    sense(s, m, st.u, st, EQ)
+    h = _host_sigma(st.u, st, m.weights())
+    d = np.abs(s - h)
+    print(f"device vs host bf16 sigma: within 2e-3 {np.mean(d <= 2e-3):.4f}, max {d.max():.3e}, "
+          f"sigma max {h.max():.3f}", flush=True)
+    assert np.mean(d <= 2e-3) >= 0.99 and d.max() <= 0.1
+
+
+def test_nonfinite_output_falls_back_to_persson(tmp_path):
+    w = MLPWeights.load()
+    w.b3 = np.array([np.nan], np.float32)
+    path = str(tmp_path / "nan.npz")
+    w.save(path)
+    st, m = _learned_case(weights=path)
+    s_l = np.zeros(st.mesh.n_elements)
+    sense(s_l, m, st.u, st, EQ)
+    st2, m2, _ = make_case(3, 37, 21, bc="transmissive", capturing="persson_br0", ic="riemann", seed=5,
+                           noise=0.01, precision="fp32_mixed")
+    s_p = np.zeros(st2.mesh.n_elements)
+    sense(s_p, m2, st2.u, st2, EQ)
+    np.testing.assert_array_equal(s_l, s_p)
+
+
+def test_learned_residual_matches_oracle():
+    st, m = _learned_case()
+    du = np.zeros(st.u.shape, order="F")
+    residual(du, st, EQ, m)
+    ref = _oracle_residual_learned(_null_oracle(st), st.u, st, m.weights())
+    err = rel_err_per_component(du, ref)
+    print(f"learned residual vs oracle with host-network viscosity: {err:.3e}", flush=True)
+    assert err <= 1e-3
+
+
+def test_learned_step_matches_oracle_stages():
+    """One step: stage 1 takes its viscosity from the element pass, stages 2 and 3 from the stage
+    kernel's epilogue on the intermediate states; the oracle composition below evaluates the
+    host network on each of those states."""
+    st, m = _learned_case(noise=0.0)
+    u0 = st.u.copy(order="F")
+    orc = _null_oracle(st)
+    w = m.weights()
+    dt = 0.2 * orc.compute_dt(u0, 0.08)
+    L = lambda u: _oracle_residual_learned(orc, u, st, w)
+    u1 = u0 + dt * L(u0)
+    u2 = 0.75 * u0 + 0.25 * (u1 + dt * L(u1))
+    u3 = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (u2 + dt * L(u2))
+    assert ssp_rk3_step(st, EQ, m, dt) == "ok"
+    err = rel_err_per_component(st.u - u0, u3 - u0)
+    print(f"learned step increment vs oracle stages: {err:.3e}", flush=True)
+    assert err <= 2e-3
+
+
+def test_constant_network_step_is_exact(tmp_path):
+    w = MLPWeights.load()
+    w.W3 = np.zeros_like(w.W3)
+    w.b3 = np.array([0.3], np.float32)
+    path = str(tmp_path / "const.npz")
+    w.save(path)
+    st, m = _learned_case(noise=0.0, weights=path)
+    u0 = st.u.copy(order="F")
+    orc = _null_oracle(st)
+    sig = np.full(st.mesh.n_elements, 0.3)
+    L = lambda u: orc.apply_br0(u, orc.element_viscosities(u, sig), orc.residual(u))
+    dt = 0.2 * orc.compute_dt(u0, 0.08)
+    u1 = u0 + dt * L(u0)
+    u2 = 0.75 * u0 + 0.25 * (u1 + dt * L(u1))
+    u3 = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (u2 + dt * L(u2))
+    assert ssp_rk3_step(st, EQ, m, dt) == "ok"
+    err = rel_err_per_component(st.u - u0, u3 - u0)
+    print(f"constant-network step increment vs oracle: {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_learned_run_and_score_delta_vs_persson():
+    """cfg6-style four-quadrant problem at p = 3 with both sensors: both runs complete, and the
+    overshoot score of the learned run is reported beside Persson's."""
+    from frforge_amd.capturing import PerssonAVMethod
+    from frforge_amd.suite2d import run_euler2d_riemann
+    out = {}
+    for name, meth in (("persson", PerssonAVMethod.create(c_av=0.1)), ("learned", LearnedSensorMethod.create())):
+        case, s, _ = run_euler2d_riemann(p=3, nx=64, ny=64, t_final=0.08, cfl=0.08, config="cfg6",
+                                         method=meth, method_name=name, precision="fp32_mixed")
+        out[name] = (case, s.u[:, :, :, 0].copy())
+        s.device.close()
+        s.device = None
+    (cp, rp), (cl, rl) = out["persson"], out["learned"]
+    l1 = float(np.sum(np.abs(rl - rp)) / np.sum(np.abs(rp)))
+    print(f"cfg6 p=3 64^2 t=0.08: overshoot persson {cp['overshoot']:.4e} learned {cl['overshoot']:.4e}; "
+          f"density L1 learned vs persson {l1:.3e}", flush=True)
+    assert cp["pass"] and cl["pass"]
+    assert l1 < 1e-2

@@ -122,8 +122,9 @@ def test_constant_network_step_is_exact(tmp_path):
 
 
 def test_learned_run_and_score_delta_vs_persson():
-    """cfg6-style four-quadrant problem at p = 3 with both sensors: both runs complete, and the
-    overshoot score of the learned run is reported beside Persson's."""
+    """Lax-Liu configuration 6 at p = 3 (64^2, t = 0.08) with both sensors: both runs complete and
+    the learned sensor's overshoot is no worse than Persson's by more than 0.01 (the training
+    data hold this configuration at 96^2 and 128^2, not at 64^2)."""
     from frforge_amd.capturing import PerssonAVMethod
     from frforge_amd.suite2d import run_euler2d_riemann
     out = {}
@@ -139,3 +140,4 @@ def test_learned_run_and_score_delta_vs_persson():
           f"density L1 learned vs persson {l1:.3e}", flush=True)
     assert cp["pass"] and cl["pass"]
     assert l1 < 1e-2
+    assert cl["overshoot"] <= cp["overshoot"] + 0.01   # measured 0.0503 against Persson's 0.0549

@@ -34,9 +34,15 @@ FLAGS = [
 ]
 
 
+# The FP64 plain form evaluates without fused multiply-adds, in the reference's operation order
+# (Julia does not contract a*b + c): at fine resolution the residual is a small difference of
+# large flux sums, so a single fused rounding moves it by more than the parity bar.
+FILE_FLAGS = {"frf_stage.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}")

@@ -50,6 +50,12 @@
 // kernel is bound by memory latency rather than by VALU issue.
 #define FRF_MFMA 0
 #endif
+#ifndef FRF_PRIO
+// Wave priority during the load phase P0 for the NullCapturing form (0: unchanged).  Measured at
+// 1024^2, p = 3: priority 2 takes the NullCapturing stage from 326 to 315 us; the Persson+BR0
+// form gets slower with it (390 -> 397 us), so it keeps the default priority.
+#define FRF_PRIO 2
+#endif
 #ifndef FRF_MWAVES
 #define FRF_MWAVES 1   // minimum waves per SIMD asked of the register allocator
 #endif
@@ -352,6 +358,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
         // earlier launches (this stage reports P.stage or 4, never less), so it is uniform.
         if (P.ctl->done != 0 || P.ctl->fail_stage < P.stage) return;
     }
+    if constexpr (FRF_PRIO > 0 && CAPT == 0) __builtin_amdgcn_s_setprio(FRF_PRIO);
     if constexpr (FRF_STAGGER > 0) {
         // The first resident blocks of a CU start together and would run their load and their
         // compute phases in lockstep.  Blocks are dispatched round-robin over the CUs, so block b
@@ -506,6 +513,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             }
         }
     }
+    if constexpr (FRF_PRIO > 0 && CAPT == 0) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
 
     // ------------------------------------------------------------------ P1: eta sweep

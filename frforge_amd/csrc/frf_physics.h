@@ -29,6 +29,13 @@ __device__ __forceinline__ double rcp_(double a) {
 }
 __device__ __forceinline__ float rcp_(float a) { return __builtin_amdgcn_rcpf(a); }
 
+// Quotient a / b: FP64 divides with IEEE rounding, as the reference does (its velocities, sound
+// speeds and unit normals feed differences that cancel, so a reciprocal-and-multiply would
+// move the FP64 residual by more than its own rounding); FP32 multiplies by the hardware
+// reciprocal.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 __device__ __forceinline__ double rsqrt_(double a) { return sqrt(a); }
 __device__ __forceinline__ float rsqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double rhypot(double a, double b) { return hypot(a, b); }
@@ -53,8 +60,8 @@ __device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
     Prim<R> q;
     q.rho = rmax(U[0], R(kEps64));
     q.inv = rcp_(q.rho);
-    q.u = U[1] * q.inv;
-    q.v = U[2] * q.inv;
+    q.u = dv(U[1], q.rho);
+    q.v = dv(U[2], q.rho);
     q.p = (gamma - R(1)) * (U[3] - R(0.5) * q.rho * (q.u * q.u + q.v * q.v));
     return q;
 }
@@ -62,7 +69,7 @@ __device__ __forceinline__ Prim<R> primitives(const R U[4], R gamma) {
 // sqrt(gamma max(p, eps) / rho) (Euler2D.jl:35-39)
 template <typename R>
 __device__ __forceinline__ R sound_speed(const Prim<R>& q, R gamma) {
-    return rsqrt_(gamma * rmax(q.p, R(kEps64)) * q.inv);
+    return rsqrt_(dv(gamma * rmax(q.p, R(kEps64)), q.rho));
 }
 
 // physical fluxes F (x) and G (y) (Euler2D.jl:54-79)
@@ -110,13 +117,12 @@ __device__ __forceinline__ void hllc_n(const R uL[4], const R uR[4], R nx0, R ny
         rusanov_n(uL, uR, nx0, ny0, gamma, out);
         return;
     }
-    const R inrm = rcp_(nrm);
-    R nx = nx0 * inrm, ny = ny0 * inrm;
+    R nx = dv(nx0, nrm), ny = dv(ny0, nrm);
     Prim<R> a = primitives(uL, gamma);
     Prim<R> b = primitives(uR, gamma);
     R pL = rmax(a.p, R(kEps64)), pR = rmax(b.p, R(kEps64));
     R unL = a.u * nx + a.v * ny, unR = b.u * nx + b.v * ny;
-    R cL = rsqrt_(gamma * pL * a.inv), cR = rsqrt_(gamma * pR * b.inv);
+    R cL = rsqrt_(dv(gamma * pL, a.rho)), cR = rsqrt_(dv(gamma * pR, b.rho));
     R SL = rmin(unL - cL, unR - cR);
     R SR = rmax(unL + cL, unR + cR);
     R num = pR - pL + a.rho * unL * (SL - unL) - b.rho * unR * (SR - unR);
@@ -205,12 +211,12 @@ __device__ __forceinline__ void hllc_axis(const R uL[4], const R uR[4], R gamma,
     const Prim<R> a = primitives(uL, gamma), b = primitives(uR, gamma);
     const R pL = rmax(a.p, R(kEps64)), pR = rmax(b.p, R(kEps64));
     const R unL = DIR == 0 ? a.u : a.v, unR = DIR == 0 ? b.u : b.v;
-    const R cL = rsqrt_(gamma * pL * a.inv), cR = rsqrt_(gamma * pR * b.inv);
+    const R cL = rsqrt_(dv(gamma * pL, a.rho)), cR = rsqrt_(dv(gamma * pR, b.rho));
     const R SL = rmin(unL - cL, unR - cR);
     const R SR = rmax(unL + cL, unR + cR);
     const R num = pR - pL + a.rho * unL * (SL - unL) - b.rho * unR * (SR - unR);
     const R den = a.rho * (SL - unL) - b.rho * (SR - unR);
-    const R SM = rabs(den) < R(kEps64) ? R(0.5) * (unL + unR) : num * rcp_(den);
+    const R SM = rabs(den) < R(kEps64) ? R(0.5) * (unL + unR) : dv(num, den);
     R fL[4], fR[4];
     axis_flux<DIR>(uL, a, fL);
     axis_flux<DIR>(uR, b, fR);
@@ -228,14 +234,13 @@ __device__ __forceinline__ void hllc_axis(const R uL[4], const R uR[4], R gamma,
     const R* U = left ? uL : uR;
     const R* Fn = left ? fL : fR;
     const R rho = left ? a.rho : b.rho;
-    const R inv = left ? a.inv : b.inv;
     const R ut = left ? (DIR == 0 ? a.v : a.u) : (DIR == 0 ? b.v : b.u);
     const R p = left ? pL : pR;
     const R un = left ? unL : unR;
     const R S = left ? SL : SR;
     const R rs = rho * (S - un);
-    const R factor = rs * rcp_(S - SM);
-    const R es = U[3] * inv + (SM - un) * (SM + p * rcp_(rs));
+    const R factor = dv(rs, S - SM);
+    const R es = dv(U[3], rho) + (SM - un) * (SM + dv(p, rs));
     out[0] = Fn[0] + S * (factor - U[0]);
     out[N] = Fn[N] + S * (factor * SM - U[N]);
     out[T] = Fn[T] + S * (factor * ut - U[T]);

@@ -36,8 +36,11 @@ struct Lane {
     __device__ __forceinline__ R operator[](int c) const { return p[c * stride]; }
 };
 
+// Two waves per SIMD are asked of the register allocator for the NullCapturing form, which
+// needs just over 256 registers without the request (one wave per SIMD otherwise).
 template <int NP, typename R, int FLUX, int CAPT, int TX, int TY, bool PF>
-__global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<NP, R> op) {
+__global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 : 1)
+    stage_kernel(StageParams P, OpsT<NP, R> op) {
     using L = TabLayout<NP>;
     constexpr int CT = L::STRIDE;
     constexpr int NT = TX * TY, NPP = NP * NP, LINE = NP * kNeq, NTHR = NT * NP;
@@ -358,8 +361,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
             const bool has_n = in_tile || jxE >= 0;
             const R a = rtR[L::FE + j], b = ctR[L::FE + j];   // y_eta, x_eta at the E face
             const R s = face_s(a, b);
-            const R is = rcp_(s);
-            const R nxo = a * is, nyo = -b * is;
+            const R nxo = dv(a, s), nyo = dv(-b, s);
             R fh[4];
             if (has_n) {
                 const Lane<R> up = in_tile ? Lane<R>{&sFx[(le + 1) * NP + j], FS} : Lane<R>{&sHE[ly * NP + j], TY * NP};
@@ -393,8 +395,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHW[c * (TY * NP) + ly * NP + j];
-                    const R isw = rcp_(sw);
-                    iflux<FLUX>(um, trW, aw * isw, -bw * isw, g, fw);
+                    iflux<FLUX>(um, trW, dv(aw, sw), dv(-bw, sw), g, fw);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[ly], eps_e, P.ctab[(size_t)jxW * CT + L::DELTA], dx, P.p);
 #pragma unroll
@@ -404,8 +405,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                 } else {
                     const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];   // own west face
                     sw = face_s(aw, bw);
-                    const R isw = rcp_(sw);
-                    const R nxw = -aw * isw, nyw = bw * isw;
+                    const R nxw = dv(-aw, sw), nyw = dv(bw, sw);
                     iflux<FLUX>(trW, trW, -nxw, -nyw, g, fw);
                     if constexpr (BR0) {
 #pragma unroll
@@ -422,8 +422,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
             const bool has_n = in_tile || jyN >= 0;
             const R a = rtR[L::FN + i], b = ctR[L::FN + i];   // y_xi, x_xi at the N face
             const R s = face_s(a, b);
-            const R is = rcp_(s);
-            const R nxo = -a * is, nyo = b * is;
+            const R nxo = dv(-a, s), nyo = dv(b, s);
             R fh[4];
             if (has_n) {
                 const Lane<R> up = in_tile ? Lane<R>{&sFy[(le + TX) * NP + i], FS} : Lane<R>{&sHN[lx * NP + i], TX * NP};
@@ -457,8 +456,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHS[c * (TX * NP) + lx * NP + i];
-                    const R iss = rcp_(ss);
-                    iflux<FLUX>(um, trS, -as * iss, bs * iss, g, fs);
+                    iflux<FLUX>(um, trS, dv(-as, ss), dv(bs, ss), g, fs);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[2 * TY + lx], eps_e, P.rtab[(size_t)jyS * CT + L::DELTA], dy, P.p);
 #pragma unroll
@@ -468,8 +466,7 @@ __global__ void __launch_bounds__(TX* TY* NP) stage_kernel(StageParams P, OpsT<N
                 } else {
                     const R as = rtR[L::FS + i], bs = ctR[L::FS + i];   // own south face
                     ss = face_s(as, bs);
-                    const R iss = rcp_(ss);
-                    const R nxs = as * iss, nys = -bs * iss;
+                    const R nxs = dv(as, ss), nys = dv(-bs, ss);
                     iflux<FLUX>(trS, trS, -nxs, -nys, g, fs);
                     if constexpr (BR0) {
 #pragma unroll

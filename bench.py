@@ -196,9 +196,29 @@ def run_cpu_baseline(nx, p, dt, capturing="persson_br0"):
             break
     el = time.perf_counter() - t0
     dof = nx * nx * (p + 1) ** 2 * 4
+    # the reference's official serial mode (with_serial_residual), one step
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    orc.rk3_step(st.u, 0.0, dt)
+    el1 = time.perf_counter() - t1
+    O.set_threads(threads)
     return {"value": dof * 3 * n / el, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
             "sample": f"{n} SSP-RK3 step(s) of the same {nx}x{nx} p={p} {capturing} FP64 workload "
-                      f"({el:.1f} s, oracle C restatement, OpenMP {threads} threads)"}
+                      f"({el:.1f} s, oracle C restatement, OpenMP {threads} threads)",
+            "serial_value": dof * 3 / el1,
+            "serial_sample": f"1 SSP-RK3 step on 1 thread ({el1:.1f} s)",
+            "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def learned_leg(ls, hs, l1, nx, p):

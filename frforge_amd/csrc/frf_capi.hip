@@ -38,6 +38,8 @@ struct frf_ctx {
     DevCtl* dCtl = nullptr;
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
     frf::MlpPacked* dMlp = nullptr;    // learned sensor weights, when set
+    double side_state[4][4] = {};      // Dirichlet ghost states (left, right, bottom, top)
+    bool side_state_set[4] = {false, false, false, false};
     HostOps ops{};
     bool ops_set = false, geom_set = false, state_set = false;
     std::vector<double> xi_nodes;     // solution points, for the metric tables
@@ -87,9 +89,9 @@ extern "C" int32_t frf_create(const frf_problem* pb, frf_ctx** out) {
     if (pb->p < 1 || pb->p > 5) return fail(nullptr, FRF_E_UNSUPPORTED, "degree p=%d not in 1..5", pb->p);
     if (pb->nx < 1 || pb->ny < 1) return fail(nullptr, FRF_E_ARG, "nx, ny must be >= 1");
     if (pb->neq != 4) return fail(nullptr, FRF_E_UNSUPPORTED, "neq=%d: only Euler2D (4) is supported", pb->neq);
-    auto okbc = [](int b) { return b == FRF_BC_PERIODIC || b == FRF_BC_TRANSMISSIVE; };
+    auto okbc = [](int b) { return b >= FRF_BC_PERIODIC && b <= FRF_BC_DIRICHLET; };
     if (!okbc(pb->bc_left) || !okbc(pb->bc_right) || !okbc(pb->bc_bottom) || !okbc(pb->bc_top))
-        return fail(nullptr, FRF_E_UNSUPPORTED, "only periodic and transmissive sides are supported");
+        return fail(nullptr, FRF_E_UNSUPPORTED, "unknown boundary kind");
     if ((pb->bc_left == FRF_BC_PERIODIC) != (pb->bc_right == FRF_BC_PERIODIC))
         return fail(nullptr, FRF_E_ARG, "x-periodic requires both left and right PeriodicBC");
     if ((pb->bc_bottom == FRF_BC_PERIODIC) != (pb->bc_top == FRF_BC_PERIODIC))
@@ -247,6 +249,20 @@ extern "C" int32_t frf_set_learned_sensor(frf_ctx* c, const void* packed, int64_
     return FRF_OK;
 }
 
+extern "C" int32_t frf_set_boundary_state(frf_ctx* c, int32_t side, const double* U) {
+    if (!c) return FRF_E_ARG;
+    if (side < 0 || side > 3 || !U) return fail(c, FRF_E_ARG, "frf_set_boundary_state: bad side or state");
+    const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
+    if (sides[side] != FRF_BC_DIRICHLET)
+        return fail(c, FRF_E_ARG, "side %d is not a Dirichlet side", side);
+    for (int q = 0; q < 4; ++q) {
+        if (!std::isfinite(U[q])) return fail(c, FRF_E_ARG, "non-finite Dirichlet state");
+        c->side_state[side][q] = U[q];
+    }
+    c->side_state_set[side] = true;
+    return FRF_OK;
+}
+
 extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* yv) {
 #pragma clang fp contract(off)
     if (!c) return FRF_E_ARG;
@@ -383,6 +399,10 @@ static int32_t ready(frf_ctx* c) {
     if (!c->ops_set) return fail(c, FRF_E_STATE, "operators not set");
     if (!c->geom_set) return fail(c, FRF_E_STATE, "geometry not set");
     if (!c->state_set) return fail(c, FRF_E_STATE, "no state uploaded");
+    const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
+    for (int k = 0; k < 4; ++k)
+        if (sides[k] == FRF_BC_DIRICHLET && !c->side_state_set[k])
+            return fail(c, FRF_E_STATE, "Dirichlet side %d has no state (frf_set_boundary_state)", k);
     return FRF_OK;
 }
 
@@ -414,6 +434,11 @@ static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double
     P.want_lambda = want_lambda;
     P.p = c->pb.p;
     P.mlp = c->dMlp;
+    const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
+    for (int k = 0; k < 4; ++k) {
+        P.side_bc[k] = sides[k];
+        for (int q = 0; q < 4; ++q) P.side_state[k][q] = c->side_state[k][q];
+    }
     return P;
 }
 

@@ -97,6 +97,8 @@ struct StageParams {
     int p;
     int resident, ncu;    // blocks of the stage kernel resident per CU, and CUs (start staggering)
     const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3, mixed form), or null
+    int side_bc[4];       // kind of the left, right, bottom, top sides (FRF_BC_*), non-periodic axes
+    double side_state[4][4];   // Dirichlet ghost states
 };
 
 struct ElemParams {

@@ -302,13 +302,27 @@ __device__ __forceinline__ void face_h(const float um[kNeq], const float up[kNeq
     }
 }
 
-// Transmissive boundary face: ghost = interior, so F^ is the physical flux of the trace and
-// g^ the interior g trace (BoundaryConditions.jl:43-132, PerssonAV2D.jl boundary branch).
-template <int DIR, bool BR0>
-__device__ __forceinline__ void boundary_h(const float ub[kNeq], const float gb[kNeq], float gamma,
-                                           float out[kNeq]) {
-    const Prim<float> q = primitives(ub, gamma);
-    axis_flux<DIR>(ub, q, out);
+// Face flux H^ = F^ - g^ on a non-periodic side of kind `kind` (FRF_BC_*), for an element on the
+// minus side of the face (E and N sides, PLUS = false) or on its plus side (W and S sides):
+// transmissive keeps F^ = F(u_b) (ghost = interior; the flux of the trace itself, so a uniform
+// state stays exact), reflecting and Dirichlet take the interface flux against the exterior
+// state (BoundaryConditions.jl:43-110, boundary_fhat! Residual2D.jl:63-94); g^ is the interior g
+// trace on every non-periodic side (PerssonAV2D.jl:310-348).
+template <int FLUX, int DIR, bool BR0, bool PLUS>
+__device__ __forceinline__ void boundary_h(int kind, const double* UD, const float ub[kNeq],
+                                           const float gb[kNeq], float gamma, float out[kNeq]) {
+    if (kind == 2 || kind == 3) {
+        float ug[kNeq];
+        const float sgn = PLUS ? -1.0f : 1.0f;   // outward normal along DIR
+        exterior_state<float>(kind, ub, DIR == 0 ? sgn : 0.0f, DIR == 0 ? 0.0f : sgn, UD, ug);
+        if constexpr (PLUS)
+            iflux_axis<FLUX, DIR>(ug, ub, gamma, out);
+        else
+            iflux_axis<FLUX, DIR>(ub, ug, gamma, out);
+    } else {
+        const Prim<float> q = primitives(ub, gamma);
+        axis_flux<DIR>(ub, q, out);
+    }
     if constexpr (BR0) {
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) out[c] -= gb[c];
@@ -575,7 +589,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                 }
                 face_h<FLUX, 0, BR0>(uE, up, gE, gp, tau, g, hE);
             } else {
-                boundary_h<0, BR0>(uE, gE, g, hE);
+                boundary_h<FLUX, 0, BR0, false>(P.side_bc[1], P.side_state[1], uE, gE, g, hE);
             }
         }
         {   // N face of column i = t
@@ -597,7 +611,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                 }
                 face_h<FLUX, 1, BR0>(uN, up, gN, gp, tau, g, hN);
             } else {
-                boundary_h<1, BR0>(uN, gN, g, hN);
+                boundary_h<FLUX, 1, BR0, false>(P.side_bc[3], P.side_state[3], uN, gN, g, hN);
             }
         }
     }
@@ -625,7 +639,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                                         ctab[(size_t)tx0 * CT + L::DELTA], P.p);
                     face_h<FLUX, 0, BR0>(um, uo, gm, go, tau, g, fh);
                 } else {
-                    boundary_h<0, BR0>(uo, go, g, fh);
+                    boundary_h<FLUX, 0, BR0, true>(P.side_bc[0], P.side_state[0], uo, go, g, fh);
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0x[c * HX + r * NP + j] = fh[c];
@@ -652,7 +666,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                                         rtab[(size_t)ty0 * CT + L::DELTA], P.p);
                     face_h<FLUX, 1, BR0>(um, uo, gm, go, tau, g, fh);
                 } else {
-                    boundary_h<1, BR0>(uo, go, g, fh);
+                    boundary_h<FLUX, 1, BR0, true>(P.side_bc[2], P.side_state[2], uo, go, g, fh);
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0y[c * HY + l * NP + i] = fh[c];

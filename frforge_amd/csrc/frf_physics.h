@@ -255,6 +255,31 @@ __device__ __forceinline__ void iflux_axis(const R uL[4], const R uR[4], R gamma
         rusanov_axis<DIR>(uL, uR, gamma, out);
 }
 
+// exterior_state of a non-periodic side (BoundaryConditions.jl:43-110) for the outward unit
+// normal (nx, ny): kind 2 reflects the velocity (reflect_conserved, with the floored density),
+// kind 3 takes the given state, anything else (transmissive) the interior trace.
+template <typename R>
+__device__ __forceinline__ void exterior_state(int kind, const R U[4], R nx, R ny, const double* UD,
+                                               R ug[4]) {
+    if (kind == 2) {
+        const R rho = rmax(U[0], R(kEps64));
+        const R u = dv(U[1], rho), v = dv(U[2], rho);
+        const R un = u * nx + v * ny;
+        const R ugx = u - R(2) * un * nx;
+        const R vgy = v - R(2) * un * ny;
+        ug[0] = rho;
+        ug[1] = rho * ugx;
+        ug[2] = rho * vgy;
+        ug[3] = U[3];
+    } else if (kind == 3) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ug[c] = R(UD[c]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ug[c] = U[c];
+    }
+}
+
 // Element wave speed term max(|u| + c, |v| + c) at one point (PerssonAV2D.jl:62-82,
 // Euler2D.jl:92-104)
 template <typename R>

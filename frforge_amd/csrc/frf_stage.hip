@@ -378,7 +378,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     for (int c = 0; c < kNeq; ++c) ghE[c] = br0_avg(gxE[c], gp[c], trE[c], upv[c], tau);
                 }
             } else {
-                iflux<FLUX>(trE, trE, nxo, nyo, g, fh);
+                R ug[4];
+                exterior_state<R>(P.side_bc[1], trE, nxo, nyo, P.side_state[1], ug);
+                iflux<FLUX>(trE, ug, nxo, nyo, g, fh);
                 if constexpr (BR0) {
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) ghE[c] = gxE[c];
@@ -406,7 +408,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];   // own west face
                     sw = face_s(aw, bw);
                     const R nxw = dv(-aw, sw), nyw = dv(bw, sw);
-                    iflux<FLUX>(trW, trW, -nxw, -nyw, g, fw);
+                    R ug[4];
+                    exterior_state<R>(P.side_bc[0], trW, nxw, nyw, P.side_state[0], ug);
+                    iflux<FLUX>(ug, trW, -nxw, -nyw, g, fw);
                     if constexpr (BR0) {
 #pragma unroll
                         for (int c = 0; c < kNeq; ++c) sG0x[c * (TY * NP) + ly * NP + j] = gxW[c];
@@ -439,7 +443,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     for (int c = 0; c < kNeq; ++c) ghN[c] = br0_avg(gyN[c], gp[c], trN[c], upv[c], tau);
                 }
             } else {
-                iflux<FLUX>(trN, trN, nxo, nyo, g, fh);
+                R ug[4];
+                exterior_state<R>(P.side_bc[3], trN, nxo, nyo, P.side_state[3], ug);
+                iflux<FLUX>(trN, ug, nxo, nyo, g, fh);
                 if constexpr (BR0) {
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) ghN[c] = gyN[c];
@@ -467,7 +473,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     const R as = rtR[L::FS + i], bs = ctR[L::FS + i];   // own south face
                     ss = face_s(as, bs);
                     const R nxs = dv(as, ss), nys = dv(-bs, ss);
-                    iflux<FLUX>(trS, trS, -nxs, -nys, g, fs);
+                    R ug[4];
+                    exterior_state<R>(P.side_bc[2], trS, nxs, nys, P.side_state[2], ug);
+                    iflux<FLUX>(ug, trS, -nxs, -nys, g, fs);
                     if constexpr (BR0) {
 #pragma unroll
                         for (int c = 0; c < kNeq; ++c) sG0y[c * (TX * NP) + lx * NP + i] = gyS[c];

@@ -15,14 +15,16 @@ LIB_PATH = os.path.join(_HERE, "libfrforge_hip.so")
 
 FRF_OK = 0
 ERRORS = {-1: "FRF_E_ARG", -2: "FRF_E_STATE", -3: "FRF_E_HIP", -4: "FRF_E_UNSUPPORTED"}
-BC = {"periodic": 0, "transmissive": 1}
+BC = {"periodic": 0, "transmissive": 1, "reflecting": 2, "dirichlet": 3}
+SIDES = {"left": 0, "right": 1, "bottom": 2, "top": 3}
 FLUX = {"rusanov": 0, "hllc": 1}
 PRECISION = {"fp64": 0, "fp32_mixed": 1}
 CAPTURING = {"null": 0, "persson_br0": 1, "persson_dd": 2}
 STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
-EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_learned_sensor",
+EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_boundary_state",
+            "frf_set_learned_sensor",
             "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
@@ -84,6 +86,7 @@ def load_library(path: str = LIB_PATH):
     L.frf_set_operators.argtypes = [vp] + [dp] * 8
     L.frf_set_geometry.argtypes = [vp, dp, dp]
     L.frf_set_learned_sensor.argtypes = [vp, C.c_void_p, C.c_int64]
+    L.frf_set_boundary_state.argtypes = [vp, C.c_int32, dp]
     L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf_residual.argtypes = [vp, dp, C.c_int64]
@@ -199,6 +202,12 @@ class DeviceContext:
         xv = np.ascontiguousarray(x_vertices, dtype=np.float64)
         yv = np.ascontiguousarray(y_vertices, dtype=np.float64)
         self._chk(self.lib.frf_set_geometry(self.h, _dp(xv), _dp(yv)), "frf_set_geometry")
+
+    def set_boundary_state(self, side: str, U):
+        Ua = np.ascontiguousarray(U, dtype=np.float64)
+        if Ua.shape != (4,):
+            raise ValueError("a Dirichlet state has 4 conserved components")
+        self._chk(self.lib.frf_set_boundary_state(self.h, SIDES[side], _dp(Ua)), "frf_set_boundary_state")
 
     def set_learned_sensor(self, packed):
         """Weights of the learned sensor (bytes from MLPWeights.pack), or None for Persson."""

@@ -34,14 +34,26 @@ class TransmissiveBC(AbstractBC):
 
 
 class ReflectingBC(AbstractBC):
-    """Slip wall (BoundaryConditions.jl:27).  Not supported by the device path."""
+    """Slip wall (BoundaryConditions.jl:27): the ghost state mirrors the velocity through the
+    outward normal (reflect_conserved, :43-60)."""
+
+    def __repr__(self):
+        return "ReflectingBC()"
 
 
 class DirichletBC(AbstractBC):
-    """Prescribed state (BoundaryConditions.jl:22).  Not supported by the device path."""
+    """Prescribed state (BoundaryConditions.jl:22).  The device path takes a constant conserved
+    state: ``DirichletBC(state=U)``; a ``u_func(x, y, t)`` of the reference's form is accepted
+    on the host side only."""
 
-    def __init__(self, u_func):
+    def __init__(self, u_func=None, state=None):
+        if u_func is None and state is None:
+            raise ValueError("DirichletBC needs a state or a u_func")
         self.u_func = u_func
+        self.state = None if state is None else tuple(float(v) for v in state)
+
+    def __repr__(self):
+        return f"DirichletBC(state={self.state})" if self.state is not None else "DirichletBC(u_func)"
 
 
 def _linspace(a: float, b: float, n: int) -> np.ndarray:

@@ -22,7 +22,7 @@ import numpy as np
 from .capturing import AbstractCapturingMethod, NullCapturing
 from .device import DeviceBatch, DeviceContext
 from .equations import Euler2D
-from .mesh import PeriodicBC, TransmissiveBC
+from .mesh import DirichletBC, PeriodicBC, ReflectingBC, TransmissiveBC
 from .state import SolutionState2D
 
 
@@ -31,7 +31,18 @@ def _bc_name(bc) -> str:
         return "periodic"
     if isinstance(bc, TransmissiveBC):
         return "transmissive"
+    if isinstance(bc, ReflectingBC):
+        return "reflecting"
+    if isinstance(bc, DirichletBC):
+        if bc.state is None:
+            raise NotImplementedError("the device path takes constant Dirichlet states "
+                                      "(DirichletBC(state=U)), not a u_func")
+        return "dirichlet"
     raise NotImplementedError(f"{type(bc).__name__} is not supported by the device path")
+
+
+def _sides(mesh):
+    return {"left": mesh.left_bc, "right": mesh.right_bc, "bottom": mesh.bottom_bc, "top": mesh.top_bc}
 
 
 def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod) -> DeviceContext:
@@ -47,7 +58,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
            _bc_name(mesh.left_bc), _bc_name(mesh.right_bc), _bc_name(mesh.bottom_bc),
            _bc_name(mesh.top_bc), state.scheme.flux, state.precision, kind,
            tuple(sorted(prm.items())), float(eq.gamma), id(state.ops),
-           getattr(method, "weights_path", None))
+           getattr(method, "weights_path", None),
+           tuple((k, b.state) for k, b in _sides(mesh).items() if isinstance(b, DirichletBC)))
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
         return dev
@@ -60,6 +72,9 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
     ops = state.ops
     dev.set_operators(ops.xi, ops.w, ops.D, ops.ell_L, ops.ell_R, ops.gL_xi, ops.gR_xi, ops.V_legendre)
     dev.set_geometry(mesh.x_vertices, mesh.y_vertices)
+    for side, bc in _sides(mesh).items():
+        if isinstance(bc, DirichletBC):
+            dev.set_boundary_state(side, bc.state)
     if getattr(method, "weights_path", None) is not None:   # learned sensor (frforge_amd.learned)
         dev.set_learned_sensor(method.weights().pack())
     dev.key = key

@@ -31,7 +31,11 @@ extern "C" {
 #define FRF_E_HIP (-3)          /* HIP runtime failure */
 #define FRF_E_UNSUPPORTED (-4)  /* configuration the device path does not implement */
 
-enum { FRF_BC_PERIODIC = 0, FRF_BC_TRANSMISSIVE = 1 };
+/* Boundary kinds (BoundaryConditions.jl): periodic (both ends of an axis), transmissive (ghost =
+ * interior trace), reflecting (slip wall: the ghost mirrors the velocity through the outward
+ * normal, reflect_conserved :43-60), Dirichlet (a constant ghost state, frf_set_boundary_state). */
+enum { FRF_BC_PERIODIC = 0, FRF_BC_TRANSMISSIVE = 1, FRF_BC_REFLECTING = 2, FRF_BC_DIRICHLET = 3 };
+enum { FRF_SIDE_LEFT = 0, FRF_SIDE_RIGHT = 1, FRF_SIDE_BOTTOM = 2, FRF_SIDE_TOP = 3 };
 enum { FRF_FLUX_RUSANOV = 0, FRF_FLUX_HLLC = 1 };
 enum { FRF_FP64 = 0, FRF_FP32_MIXED = 1 };
 enum { FRF_CAPT_NULL = 0, FRF_CAPT_PERSSON_BR0 = 1, FRF_CAPT_PERSSON_DD = 2 };
@@ -76,6 +80,11 @@ int32_t frf_set_operators(frf_ctx* ctx, const double* xi, const double* w, const
 /* Mesh vertices (Mesh2D.x_vertices, nx+1 values; y_vertices, ny+1 values).  The library
  * rebuilds the affine metric terms of build_mesh_metrics (MeshMetrics2D.jl:82-218) from them. */
 int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y_vertices);
+
+/* The conserved state U[4] of a FRF_BC_DIRICHLET side (FRF_SIDE_*): exterior_state for
+ * DirichletBC with a constant u_func (BoundaryConditions.jl:90-110).  Required before a run for
+ * every Dirichlet side. */
+int32_t frf_set_boundary_state(frf_ctx* ctx, int32_t side, const double* U);
 
 /* Learned troubled-cell sensor in place of the Persson sigma, for the sense! hook of a Persson
  * AV method (Interfaces.jl:79-87, PerssonAV2D.jl:20-59; the viscosity formula of

@@ -168,6 +168,8 @@ int orc_build_operators(int p, double* xi, double* w, double* D, double* lL, dou
 
 struct orc_ctx {
     orc_problem pb;
+    int bc_kind[4];                   /* per side: left, right, bottom, top */
+    double bc_state[4][4];            /* Dirichlet ghost states */
     int Np, Nel;
     double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
     double *LU; int* piv;             /* LU of V with partial pivoting (sensor solves) */
@@ -313,6 +315,8 @@ orc_ctx* orc_create(const orc_problem* pb, const double* xi, const double* w, co
     if (pb->p < 0 || pb->nx < 1 || pb->ny < 1) return NULL;
     orc_ctx* h = (orc_ctx*)calloc(1, sizeof(orc_ctx));
     h->pb = *pb;
+    h->bc_kind[0] = h->bc_kind[1] = pb->bc_x;
+    h->bc_kind[2] = h->bc_kind[3] = pb->bc_y;
     int Np = pb->p + 1, Nel = pb->nx * pb->ny;
     h->Np = Np; h->Nel = Nel;
     h->xi = dup_arr(xi, Np); h->w = dup_arr(w, Np); h->D = dup_arr(D, Np * Np);
@@ -453,12 +457,47 @@ void orc_flux_n(int kind, double gamma, const double* uL, const double* uR, doub
 
 /* ---------------------------------------------------------------- residual */
 
-/* boundary_fhat! for TransmissiveBC (Residual2D.jl:63-94; ghost = interior, BoundaryConditions.jl:67-77) */
+int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U) {
+    if (side < 0 || side > 3) return -1;
+    int axis_bc = side < 2 ? h->pb.bc_x : h->pb.bc_y;
+    if (axis_bc == ORC_BC_PERIODIC || kind < ORC_BC_TRANSMISSIVE || kind > ORC_BC_DIRICHLET) return -1;
+    if (kind == ORC_BC_DIRICHLET && !U) return -1;
+    h->bc_kind[side] = kind;
+    if (kind == ORC_BC_DIRICHLET) memcpy(h->bc_state[side], U, sizeof h->bc_state[side]);
+    return 0;
+}
+
+/* exterior_state (BoundaryConditions.jl:64-110): transmissive copies the interior trace,
+ * reflecting mirrors the velocity through the outward unit normal with the floored density
+ * (reflect_conserved, :43-60), Dirichlet takes the given state. */
+static void exterior_state(const orc_ctx* h, int side, const double* U, double nxo, double nyo,
+                           double* ug) {
+    int kind = h->bc_kind[side];
+    if (kind == ORC_BC_REFLECTING) {
+        double rho = jmax(U[0], EPS);
+        double u = U[1] / rho;
+        double v = U[2] / rho;
+        double un = u * nxo + v * nyo;
+        double ugx = u - 2 * un * nxo;
+        double vgy = v - 2 * un * nyo;
+        ug[0] = rho;
+        ug[1] = rho * ugx;
+        ug[2] = rho * vgy;
+        ug[3] = U[3];
+    } else if (kind == ORC_BC_DIRICHLET) {
+        memcpy(ug, h->bc_state[side], 4 * sizeof(double));
+    } else {
+        memcpy(ug, U, 4 * sizeof(double));
+    }
+}
+
+/* boundary_fhat! (Residual2D.jl:63-94): the interface flux between the interior trace and the
+ * side's exterior state, oriented along the reference direction (ghost first on W/S faces) */
 static void boundary_fhat(const orc_ctx* h, double* fhat, int q, int e, const double* uint_,
-                          double nxo, double nyo, double sJ, int into_domain) {
+                          double nxo, double nyo, double sJ, int into_domain, int side) {
     int Np = h->Np;
     double ug[4], fh[4];
-    memcpy(ug, uint_, sizeof ug);
+    exterior_state(h, side, uint_, nxo, nyo, ug);
     if (into_domain) iflux(h, ug, uint_, -nxo, -nyo, fh);
     else iflux(h, uint_, ug, nxo, nyo, fh);
     for (int c = 0; c < 4; ++c) fhat[FIDX(q, c, e)] = fh[c] * sJ;
@@ -541,10 +580,10 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
             } else {
                 get_face(h, trW, eL, q, um);
                 boundary_fhat(h, fW, q, eL, um, met[7][MIDX(q, eL)], met[8][MIDX(q, eL)],
-                              met[9][MIDX(q, eL)], 1);
+                              met[9][MIDX(q, eL)], 1, ORC_SIDE_LEFT);
                 get_face(h, trE, eR, q, up);
                 boundary_fhat(h, fE, q, eR, up, met[10][MIDX(q, eR)], met[11][MIDX(q, eR)],
-                              met[12][MIDX(q, eR)], 0);
+                              met[12][MIDX(q, eR)], 0, ORC_SIDE_RIGHT);
             }
         }
     }
@@ -585,10 +624,10 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
             } else {
                 get_face(h, trS, eB, q, um);
                 boundary_fhat(h, fS, q, eB, um, met[13][MIDX(q, eB)], met[14][MIDX(q, eB)],
-                              met[15][MIDX(q, eB)], 1);
+                              met[15][MIDX(q, eB)], 1, ORC_SIDE_BOTTOM);
                 get_face(h, trN, eT, q, up);
                 boundary_fhat(h, fN, q, eT, up, met[16][MIDX(q, eT)], met[17][MIDX(q, eT)],
-                              met[18][MIDX(q, eT)], 0);
+                              met[18][MIDX(q, eT)], 0, ORC_SIDE_TOP);
             }
         }
     }

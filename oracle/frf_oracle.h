@@ -22,7 +22,8 @@
 extern "C" {
 #endif
 
-enum { ORC_BC_PERIODIC = 0, ORC_BC_TRANSMISSIVE = 1 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_TRANSMISSIVE = 1, ORC_BC_REFLECTING = 2, ORC_BC_DIRICHLET = 3 };
+enum { ORC_SIDE_LEFT = 0, ORC_SIDE_RIGHT = 1, ORC_SIDE_BOTTOM = 2, ORC_SIDE_TOP = 3 };
 enum { ORC_FLUX_RUSANOV = 0, ORC_FLUX_HLLC = 1 };
 enum { ORC_CAPT_NULL = 0, ORC_CAPT_PERSSON_BR0 = 1, ORC_CAPT_PERSSON_DD = 2 };
 enum { ORC_OK = 0, ORC_DIVERGED = 1, ORC_MAX_STEPS = 2 };
@@ -57,6 +58,11 @@ int orc_get_threads(void);
  * (Np*Nel); 19 = h_char (Nel). Returns pointer into the context. */
 const double* orc_metric(orc_ctx* h, int which);
 
+/* Boundary kind of one non-periodic side (default transmissive): ORC_BC_TRANSMISSIVE,
+ * ORC_BC_REFLECTING or ORC_BC_DIRICHLET with the constant conserved state U[4]
+ * (exterior_state, BoundaryConditions.jl:64-110).  Returns 0, or -1 for a periodic axis or an
+ * unknown kind. */
+int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U);
 /* residual!(du, state, eq, method) -- Residual2D.jl:131-416 */
 void orc_residual(orc_ctx* h, const double* u, double t, double* du);
 /* sense! (PerssonAV2D.jl:20-59): sigma[Nel] */

@@ -60,6 +60,8 @@ def lib():
         L.orc_metric.restype = dp
         L.orc_metric.argtypes = [C.c_void_p, C.c_int]
         L.orc_residual.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.orc_set_boundary.restype = C.c_int
+        L.orc_set_boundary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_sense.argtypes = [C.c_void_p, dp, dp]
         L.orc_element_viscosities.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_apply_br0_eps.argtypes = [C.c_void_p, dp, dp, dp]
@@ -125,9 +127,14 @@ class Oracle:
     """One oracle problem: operators (xi, w, D, lL, lR, gLxi, gRxi, V as numpy arrays with
     matrices indexed [i, j]) and the mesh vertices are inputs."""
 
+    BC_KIND = {"transmissive": 1, "reflecting": 2, "dirichlet": 3}
+    SIDE = {"left": 0, "right": 1, "bottom": 2, "top": 3}
+
     def __init__(self, p, nx, ny, xv, yv, ops, bc_x="periodic", bc_y="periodic",
                  flux="rusanov", capturing="null", gamma=1.4, kappa=4.0, s0_factor=-4.0,
-                 eps_floor=1e-16, c_av=0.1):
+                 eps_floor=1e-16, c_av=0.1, sides=None):
+        """sides: optional {"left"|"right"|"bottom"|"top": "transmissive" | "reflecting" |
+        ("dirichlet", U[4])} for the non-periodic axes."""
         pb = OrcProblem(p, nx, ny, BC_PERIODIC if bc_x == "periodic" else BC_TRANSMISSIVE,
                         BC_PERIODIC if bc_y == "periodic" else BC_TRANSMISSIVE, FLUX[flux],
                         CAPT[capturing], gamma, kappa, s0_factor, eps_floor, c_av)
@@ -142,6 +149,14 @@ class Oracle:
                                   _p(self._xv), _p(self._yv))
         if not self.h:
             raise ValueError("orc_create failed")
+        self._bc_states = []
+        for side, spec in (sides or {}).items():
+            kind, U = (spec, None) if isinstance(spec, str) else spec
+            Ua = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+            self._bc_states.append(Ua)
+            if lib().orc_set_boundary(self.h, self.SIDE[side], self.BC_KIND[kind],
+                                      None if Ua is None else _p(Ua)) != 0:
+                raise ValueError(f"orc_set_boundary({side}, {kind}) failed")
 
     def __del__(self):
         if getattr(self, "h", None):

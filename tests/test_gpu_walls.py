@@ -1,0 +1,89 @@
+"""Reflecting and Dirichlet sides on the GPU against the oracle (whose wall fluxes
+tests/test_oracle_walls.py pins): per-stage residuals of the FP64 form within 1e-12 and of the
+FP32-mixed form within 1e-4 (normwise per component), and multi-step runs of a closed box."""
+import numpy as np
+import pytest
+
+from frforge_amd import (Mesh2D, SchemeConfig, TransmissiveBC, allocate_state, build_operators,
+                         residual, set_initial_condition, ssp_rk3)
+from frforge_amd.equations import primitives_to_conserved
+from frforge_amd.mesh import DirichletBC, ReflectingBC
+from oracle.oracle import Oracle
+
+from helpers import EQ, method_for, oracle_ops, rel_err_per_component
+
+pytestmark = pytest.mark.gpu
+
+U_IN = primitives_to_conserved(EQ, 1.4, 0.6, 0.1, 1.2)
+
+
+def _ic(x, y):
+    rho = np.where(x < 0.3 + 0.2 * y, 2.0, 1.0) + 0.05 * np.sin(3 * x + 2 * y)
+    u = 0.3 * np.cos(np.pi * y) + 0.1
+    v = -0.2 * np.sin(np.pi * x)
+    p = np.where(x < 0.3 + 0.2 * y, 2.0, 1.0) * (1.0 + 0.05 * np.cos(2 * (x + y)))
+    return primitives_to_conserved(EQ, rho, u, v, p)
+
+
+SETUPS = {
+    "box": {"left": "reflecting", "right": "reflecting", "bottom": "reflecting", "top": "reflecting"},
+    "channel": {"left": ("dirichlet", U_IN), "right": "transmissive", "bottom": "reflecting",
+                "top": "reflecting"},
+}
+
+
+def _bc(spec):
+    if spec == "reflecting":
+        return ReflectingBC()
+    if spec == "transmissive":
+        return TransmissiveBC()
+    return DirichletBC(state=spec[1])
+
+
+def _case(p, setup, capturing, flux, precision, nx=37, ny=21):
+    sides = SETUPS[setup]
+    mesh = Mesh2D(0.0, 1.0, 0.0, 1.0, nx, ny, left_bc=_bc(sides["left"]), right_bc=_bc(sides["right"]),
+                  bottom_bc=_bc(sides["bottom"]), top_bc=_bc(sides["top"]))
+    ops = build_operators(p)
+    st = allocate_state(mesh, ops, scheme=SchemeConfig(flux=flux), precision=precision)
+    set_initial_condition(st, _ic)
+    m = method_for(capturing)
+    orc = Oracle(p, nx, ny, mesh.x_vertices, mesh.y_vertices, oracle_ops(ops), bc_x="transmissive",
+                 bc_y="transmissive", flux=flux, capturing=capturing, sides=sides)
+    return st, m, orc
+
+
+@pytest.mark.parametrize("capturing", ["null", "persson_br0", "persson_dd"])
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+@pytest.mark.parametrize("setup", ["box", "channel"])
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_wall_residual_fp64(p, setup, flux, capturing):
+    st, m, orc = _case(p, setup, capturing, flux, "fp64")
+    du = np.zeros(st.u.shape, order="F")
+    residual(du, st, EQ, m)
+    err = rel_err_per_component(du, orc.residual(st.u))
+    print(f"walls fp64 p={p} {setup} {flux} {capturing}: {err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("capturing", ["null", "persson_br0"])
+@pytest.mark.parametrize("setup", ["box", "channel"])
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_wall_residual_mixed(p, setup, capturing):
+    st, m, orc = _case(p, setup, capturing, "rusanov", "fp32_mixed")
+    du = np.zeros(st.u.shape, order="F")
+    residual(du, st, EQ, m)
+    err = rel_err_per_component(du, orc.residual(st.u))
+    print(f"walls mixed p={p} {setup} {capturing}: {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("setup", ["box", "channel"])
+def test_wall_run_fp64_matches_oracle(setup):
+    st, m, orc = _case(3, setup, "persson_br0", "rusanov", "fp64", nx=24, ny=16)
+    st_o, n_o, t_o, u_o = orc.run(st.u, 0.0, 0.02, cfl=0.08)
+    r = ssp_rk3(st, EQ, m, 0.02, cfl=0.08)
+    err = rel_err_per_component(st.u, u_o)
+    print(f"walls run {setup}: n={r['n_steps']} (oracle {n_o}) rel err {err:.3e}", flush=True)
+    assert (r["status"], r["n_steps"]) == (st_o, n_o) and r["t"] == t_o
+    assert err <= 1e-11

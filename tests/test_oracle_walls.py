@@ -1,0 +1,137 @@
+"""Reflecting and Dirichlet boundaries of the oracle (exterior_state, BoundaryConditions.jl:43-110;
+boundary_fhat!, Residual2D.jl:63-94), pinned by what the mathematics fixes:
+
+* a closed box (four reflecting walls) conserves mass and energy: sum w_i w_j |J| du = 0 for the
+  density and the energy (the wall flux of both vanishes when the ghost mirrors the velocity);
+* mirror symmetry: on a state symmetric about x = 1/2 in a periodic domain, the interface at
+  x = 1/2 sees the mirrored trace, so the half domain [1/2, 1] with reflecting walls at both x
+  ends has the full domain's residual restricted to that half;
+* a Dirichlet side changes the residual only in the elements on that side, by the correction
+  -(F^_D - F^_T) sJ g'_L / J with F^_D the flux against the given state and F^_T the
+  transmissive one.
+"""
+import numpy as np
+import pytest
+
+from frforge_amd import (Mesh2D, PeriodicBC, TransmissiveBC, allocate_state, build_operators,
+                         set_initial_condition)
+from frforge_amd.equations import primitives_to_conserved
+from oracle.oracle import Oracle, flux_n
+
+from helpers import EQ, oracle_ops
+
+WALLS = {"left": "reflecting", "right": "reflecting", "bottom": "reflecting", "top": "reflecting"}
+
+
+def _smooth(x, y):
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * x) * np.cos(np.pi * y) + 0.1 * x
+    u = 0.3 * np.cos(np.pi * y) + 0.2 * x
+    v = -0.2 * np.sin(np.pi * x) + 0.1
+    p = 1.0 + 0.1 * np.cos(2 * np.pi * (x + y))
+    return primitives_to_conserved(EQ, rho, u, v, p)
+
+
+def _case(p, nx, ny, x0=0.0, x1=1.0, periodic_x=False, periodic_y=False, f=_smooth):
+    ops = build_operators(p)
+    Bx = PeriodicBC if periodic_x else TransmissiveBC
+    By = PeriodicBC if periodic_y else TransmissiveBC
+    mesh = Mesh2D(x0, x1, 0.0, 1.0, nx, ny, left_bc=Bx(), right_bc=Bx(), bottom_bc=By(), top_bc=By())
+    st = allocate_state(mesh, ops)
+    set_initial_condition(st, f)
+    return st
+
+
+def _oracle(st, capturing="null", flux="rusanov", sides=None):
+    m = st.mesh
+    bx = "periodic" if isinstance(m.left_bc, PeriodicBC) else "transmissive"
+    by = "periodic" if isinstance(m.bottom_bc, PeriodicBC) else "transmissive"
+    return Oracle(st.p, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(st.ops), bc_x=bx, bc_y=by,
+                  flux=flux, capturing=capturing, sides=sides)
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+@pytest.mark.parametrize("p", [1, 3])
+def test_closed_box_conserves_mass_and_energy(p, flux):
+    """Inviscid: BR0 keeps the reference's boundary viscous flux g^ = g_interior on every
+    non-periodic side (PerssonAV2D.jl:310-348), which is not zero at a wall."""
+    st = _case(p, 9, 7)
+    orc = _oracle(st, "null", flux, WALLS)
+    du = orc.residual(st.u)
+    w = st.ops.w
+    W = w[:, None, None] * w[None, :, None] * np.abs(st.metrics.J)
+    for c in (0, 3):
+        tot = np.sum(W * du[:, :, :, c])
+        scale = np.sum(W * np.abs(du[:, :, :, c]))
+        assert abs(tot) <= 1e-13 * scale, (c, tot, scale)
+    # the x momentum is not conserved: the walls push
+    assert abs(np.sum(W * du[:, :, :, 1])) > 1e-6 * np.sum(W * np.abs(du[:, :, :, 1]))
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+@pytest.mark.parametrize("p", [2, 3])
+def test_mirror_symmetry_matches_reflecting_walls(p, flux):
+    m = 6
+
+    def sym(x, y):   # symmetric about x = 1/2 (and so about x = 0 = 1): rho, v, p even, u odd
+        s = np.abs(x - 0.5)
+        rho = 1.0 + 0.2 * np.cos(2 * np.pi * s) + 0.1 * np.sin(2 * np.pi * y)
+        u = 0.3 * np.sin(2 * np.pi * (x - 0.5))
+        v = 0.2 * np.cos(2 * np.pi * s) + 0.05 * np.cos(2 * np.pi * y)
+        pr = 1.0 + 0.1 * np.cos(4 * np.pi * s)
+        return primitives_to_conserved(EQ, rho, u, v, pr)
+
+    full = _case(p, 2 * m, 5, periodic_x=True, periodic_y=True, f=sym)
+    half = _case(p, m, 5, x0=0.5, x1=1.0, periodic_y=True, f=sym)
+    du_full = _oracle(full, flux=flux).residual(full.u)
+    du_half = _oracle(half, flux=flux, sides={"left": "reflecting", "right": "reflecting"}).residual(half.u)
+    nx = 2 * m
+    right = [jy * nx + jx for jy in range(5) for jx in range(m, nx)]
+    ref = du_full[:, :, right, :]
+    err = np.max(np.abs(du_half - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-11, err
+
+
+@pytest.mark.parametrize("side", ["left", "top"])
+def test_dirichlet_side_changes_only_its_elements(side):
+    p, nx, ny = 3, 5, 4
+    st = _case(p, nx, ny)
+    UD = primitives_to_conserved(EQ, 1.3, 0.4, -0.1, 1.2)
+    du_t = _oracle(st).residual(st.u)
+    du_d = _oracle(st, sides={side: ("dirichlet", UD)}).residual(st.u)
+    orc = _oracle(st)
+    Np, ops = p + 1, st.ops
+    # metric arrays: 7..9 W (nx, ny, sJ), 16..18 N; J = 6
+    J = orc.metric(6).reshape(Np, Np, nx * ny, order="F")
+    pred = np.zeros_like(du_t)
+    u = st.u
+    if side == "left":
+        nxw, nyw, sJ = (orc.metric(k).reshape(Np, nx * ny, order="F") for k in (7, 8, 9))
+        for jy in range(ny):
+            e = jy * nx
+            for j in range(Np):
+                tr = np.array([np.dot(ops.ell_L, u[:, j, e, c]) for c in range(4)])
+                fd = flux_n("rusanov", 1.4, UD, tr, -nxw[j, e], -nyw[j, e]) * sJ[j, e]
+                ft = flux_n("rusanov", 1.4, tr, tr, -nxw[j, e], -nyw[j, e]) * sJ[j, e]
+                for i in range(Np):
+                    pred[i, j, e, :] = -((fd - ft) * ops.gL_xi[i]) / J[i, j, e]
+    else:
+        nxn, nyn, sJ = (orc.metric(k).reshape(Np, nx * ny, order="F") for k in (16, 17, 18))
+        for jx in range(nx):
+            e = (ny - 1) * nx + jx
+            for i in range(Np):
+                tr = np.array([np.dot(ops.ell_R, u[i, :, e, c]) for c in range(4)])
+                fd = flux_n("rusanov", 1.4, tr, UD, nxn[i, e], nyn[i, e]) * sJ[i, e]
+                ft = flux_n("rusanov", 1.4, tr, tr, nxn[i, e], nyn[i, e]) * sJ[i, e]
+                for j in range(Np):
+                    pred[i, j, e, :] = -((fd - ft) * ops.gR_xi[j]) / J[i, j, e]
+    diff = du_d - du_t
+    scale = np.max(np.abs(pred))
+    assert scale > 1e-3
+    assert np.max(np.abs(diff - pred)) <= 1e-11 * max(scale, np.max(np.abs(du_t)))
+
+
+def test_reflecting_rest_state_is_steady():
+    """A fluid at rest in a closed box: zero residual (the wall flux is the pressure alone)."""
+    st = _case(3, 6, 5, f=lambda x, y: primitives_to_conserved(EQ, 1.0 + 0 * x, 0 * x, 0 * x, 1.0 + 0 * x))
+    du = _oracle(st, sides=WALLS).residual(st.u)
+    assert np.max(np.abs(du)) < 1e-12

@@ -169,6 +169,7 @@ int orc_build_operators(int p, double* xi, double* w, double* D, double* lL, dou
 struct orc_ctx {
     orc_problem pb;
     int bc_kind[4];                   /* per side: left, right, bottom, top */
+    unsigned char* solid;             /* solid-element mask (Mesh2D.solid), or NULL */
     double bc_state[4][4];            /* Dirichlet ghost states */
     int Np, Nel;
     double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
@@ -347,6 +348,7 @@ orc_ctx* orc_create(const orc_problem* pb, const double* xi, const double* w, co
 
 void orc_destroy(orc_ctx* h) {
     if (!h) return;
+    free(h->solid);
     double* all[] = {h->xi, h->w, h->D, h->lL, h->lR, h->gLxi, h->gRxi, h->V, h->LU, h->xv, h->yv,
                      h->dx, h->dy, h->Jx, h->Jy, h->trW, h->trE, h->trS, h->trN, h->fW, h->fE,
                      h->fS, h->fN, h->sigma, h->eps, h->gx, h->gy, h->uW, h->uE, h->uS, h->uN,
@@ -457,6 +459,20 @@ void orc_flux_n(int kind, double gamma, const double* uL, const double* uR, doub
 
 /* ---------------------------------------------------------------- residual */
 
+int orc_set_solid(orc_ctx* h, const unsigned char* mask) {
+    if (!mask) {
+        free(h->solid);
+        h->solid = NULL;
+        return 0;
+    }
+    if (h->pb.bc_x == ORC_BC_PERIODIC || h->pb.bc_y == ORC_BC_PERIODIC) return -1;
+    if (!h->solid) h->solid = (unsigned char*)malloc((size_t)h->Nel);
+    for (int e = 0; e < h->Nel; ++e) h->solid[e] = mask[e] ? 1 : 0;
+    return 0;
+}
+
+static inline int is_solid(const orc_ctx* h, int e) { return h->solid && h->solid[e]; }
+
 int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U) {
     if (side < 0 || side > 3) return -1;
     int axis_bc = side < 2 ? h->pb.bc_x : h->pb.bc_y;
@@ -472,7 +488,7 @@ int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U) {
  * (reflect_conserved, :43-60), Dirichlet takes the given state. */
 static void exterior_state(const orc_ctx* h, int side, const double* U, double nxo, double nyo,
                            double* ug) {
-    int kind = h->bc_kind[side];
+    int kind = side < 0 ? ORC_BC_REFLECTING : h->bc_kind[side];   /* side -1: solid wall */
     if (kind == ORC_BC_REFLECTING) {
         double rho = jmax(U[0], EPS);
         double u = U[1] / rho;
@@ -549,9 +565,22 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
     for (int f = 0; f < nvf; ++f) {
         int jy = f / (nx - 1), jx = f % (nx - 1);
         int eL = jy * nx + jx, eR = eL + 1;
+        int sL = is_solid(h, eL), sR = is_solid(h, eR);
+        if (sL && sR) continue;
         double um[4], up[4], fh[4];
         for (int q = 0; q < Np; ++q) {
             double nnx = met[10][MIDX(q, eL)], nny = met[11][MIDX(q, eL)], sJ = met[12][MIDX(q, eL)];
+            if (sL) {          /* wall on the west face of eR (Residual2D.jl:183-191) */
+                get_face(h, trW, eR, q, um);
+                boundary_fhat(h, fW, q, eR, um, met[7][MIDX(q, eR)], met[8][MIDX(q, eR)],
+                              met[9][MIDX(q, eR)], 1, -1);
+                continue;
+            }
+            if (sR) {          /* wall on the east face of eL (:192-198) */
+                get_face(h, trE, eL, q, um);
+                boundary_fhat(h, fE, q, eL, um, nnx, nny, sJ, 0, -1);
+                continue;
+            }
             get_face(h, trE, eL, q, um);
             get_face(h, trW, eR, q, up);
             iflux(h, um, up, nnx, nny, fh);
@@ -578,12 +607,16 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
                     fW[FIDX(q, c, eL)] = val;
                 }
             } else {
-                get_face(h, trW, eL, q, um);
-                boundary_fhat(h, fW, q, eL, um, met[7][MIDX(q, eL)], met[8][MIDX(q, eL)],
-                              met[9][MIDX(q, eL)], 1, ORC_SIDE_LEFT);
-                get_face(h, trE, eR, q, up);
-                boundary_fhat(h, fE, q, eR, up, met[10][MIDX(q, eR)], met[11][MIDX(q, eR)],
-                              met[12][MIDX(q, eR)], 0, ORC_SIDE_RIGHT);
+                if (!is_solid(h, eL)) {
+                    get_face(h, trW, eL, q, um);
+                    boundary_fhat(h, fW, q, eL, um, met[7][MIDX(q, eL)], met[8][MIDX(q, eL)],
+                                  met[9][MIDX(q, eL)], 1, ORC_SIDE_LEFT);
+                }
+                if (!is_solid(h, eR)) {
+                    get_face(h, trE, eR, q, up);
+                    boundary_fhat(h, fE, q, eR, up, met[10][MIDX(q, eR)], met[11][MIDX(q, eR)],
+                                  met[12][MIDX(q, eR)], 0, ORC_SIDE_RIGHT);
+                }
             }
         }
     }
@@ -593,9 +626,22 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
     for (int f = 0; f < nhf; ++f) {
         int jy = f / nx, jx = f % nx;
         int eB = jy * nx + jx, eT = eB + nx;
+        int sB = is_solid(h, eB), sT = is_solid(h, eT);
+        if (sB && sT) continue;
         double um[4], up[4], fh[4];
         for (int q = 0; q < Np; ++q) {
             double nnx = met[16][MIDX(q, eB)], nny = met[17][MIDX(q, eB)], sJ = met[18][MIDX(q, eB)];
+            if (sB) {          /* wall on the south face of eT (Residual2D.jl:265-273) */
+                get_face(h, trS, eT, q, um);
+                boundary_fhat(h, fS, q, eT, um, met[13][MIDX(q, eT)], met[14][MIDX(q, eT)],
+                              met[15][MIDX(q, eT)], 1, -1);
+                continue;
+            }
+            if (sT) {          /* wall on the north face of eB */
+                get_face(h, trN, eB, q, um);
+                boundary_fhat(h, fN, q, eB, um, nnx, nny, sJ, 0, -1);
+                continue;
+            }
             get_face(h, trN, eB, q, um);
             get_face(h, trS, eT, q, up);
             iflux(h, um, up, nnx, nny, fh);
@@ -622,12 +668,16 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
                     fS[FIDX(q, c, eB)] = val;
                 }
             } else {
-                get_face(h, trS, eB, q, um);
-                boundary_fhat(h, fS, q, eB, um, met[13][MIDX(q, eB)], met[14][MIDX(q, eB)],
-                              met[15][MIDX(q, eB)], 1, ORC_SIDE_BOTTOM);
-                get_face(h, trN, eT, q, up);
-                boundary_fhat(h, fN, q, eT, up, met[16][MIDX(q, eT)], met[17][MIDX(q, eT)],
-                              met[18][MIDX(q, eT)], 0, ORC_SIDE_TOP);
+                if (!is_solid(h, eB)) {
+                    get_face(h, trS, eB, q, um);
+                    boundary_fhat(h, fS, q, eB, um, met[13][MIDX(q, eB)], met[14][MIDX(q, eB)],
+                                  met[15][MIDX(q, eB)], 1, ORC_SIDE_BOTTOM);
+                }
+                if (!is_solid(h, eT)) {
+                    get_face(h, trN, eT, q, up);
+                    boundary_fhat(h, fN, q, eT, up, met[16][MIDX(q, eT)], met[17][MIDX(q, eT)],
+                                  met[18][MIDX(q, eT)], 0, ORC_SIDE_TOP);
+                }
             }
         }
     }
@@ -639,6 +689,7 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
         double* Gt = (double*)malloc(sizeof(double) * Np * Np * NEQ);
 #pragma omp for schedule(static)
         for (int e = 0; e < Nel; ++e) {
+            if (is_solid(h, e)) continue;   /* Residual2D.jl:338 */
             double U[4], Fx[4], Gy[4];
             for (int j = 0; j < Np; ++j)
                 for (int i = 0; i < Np; ++i) {
@@ -687,12 +738,22 @@ void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
         free(Gt);
     }
 
-    /* sense! + apply_dissipation! (:400-407) */
-    if (h->pb.capturing == ORC_CAPT_NULL) return;
-    orc_sense(h, u, h->sigma);
-    orc_element_viscosities(h, u, h->sigma, h->eps);
-    if (h->pb.capturing == ORC_CAPT_PERSSON_BR0) apply_br0(h, u, h->eps, du);
-    else apply_dd(h, u, h->eps, du);
+    /* sense! + apply_dissipation! (:400-407); solid elements: sigma = 0, du = 0 (:402-414) */
+    if (h->pb.capturing != ORC_CAPT_NULL) {
+        orc_sense(h, u, h->sigma);
+        if (h->solid)
+            for (int e = 0; e < Nel; ++e)
+                if (h->solid[e]) h->sigma[e] = 0.0;
+        orc_element_viscosities(h, u, h->sigma, h->eps);
+        if (h->pb.capturing == ORC_CAPT_PERSSON_BR0) apply_br0(h, u, h->eps, du);
+        else apply_dd(h, u, h->eps, du);
+    }
+    if (h->solid)
+        for (int e = 0; e < Nel; ++e)
+            if (h->solid[e])
+                for (int c = 0; c < NEQ; ++c)
+                    for (int j = 0; j < Np; ++j)
+                        for (int i = 0; i < Np; ++i) du[IDX4(i, j, e, c)] = 0.0;
 }
 
 /* ---------------------------------------------------------------- capturing */

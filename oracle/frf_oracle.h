@@ -63,6 +63,12 @@ const double* orc_metric(orc_ctx* h, int which);
  * (exterior_state, BoundaryConditions.jl:64-110).  Returns 0, or -1 for a periodic axis or an
  * unknown kind. */
 int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U);
+/* Solid-element mask (Mesh2D.solid, Mesh2D.jl:31-95): faces between a fluid and a solid element
+ * are reflecting walls of the fluid element, solid elements get sigma = 0 and du = 0
+ * (Residual2D.jl:140-416).  mask[Nel] nonzero = solid; NULL clears it.  Non-periodic axes only
+ * (the reference leaves the flux of a periodic wrap face next to a solid element from its
+ * previous evaluation).  Returns 0 or -1. */
+int orc_set_solid(orc_ctx* h, const unsigned char* mask);
 /* residual!(du, state, eq, method) -- Residual2D.jl:131-416 */
 void orc_residual(orc_ctx* h, const double* u, double t, double* du);
 /* sense! (PerssonAV2D.jl:20-59): sigma[Nel] */

@@ -60,6 +60,8 @@ def lib():
         L.orc_metric.restype = dp
         L.orc_metric.argtypes = [C.c_void_p, C.c_int]
         L.orc_residual.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.orc_set_solid.restype = C.c_int
+        L.orc_set_solid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.orc_set_boundary.restype = C.c_int
         L.orc_set_boundary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_sense.argtypes = [C.c_void_p, dp, dp]
@@ -132,7 +134,7 @@ class Oracle:
 
     def __init__(self, p, nx, ny, xv, yv, ops, bc_x="periodic", bc_y="periodic",
                  flux="rusanov", capturing="null", gamma=1.4, kappa=4.0, s0_factor=-4.0,
-                 eps_floor=1e-16, c_av=0.1, sides=None):
+                 eps_floor=1e-16, c_av=0.1, sides=None, solid=None):
         """sides: optional {"left"|"right"|"bottom"|"top": "transmissive" | "reflecting" |
         ("dirichlet", U[4])} for the non-periodic axes."""
         pb = OrcProblem(p, nx, ny, BC_PERIODIC if bc_x == "periodic" else BC_TRANSMISSIVE,
@@ -157,6 +159,12 @@ class Oracle:
             if lib().orc_set_boundary(self.h, self.SIDE[side], self.BC_KIND[kind],
                                       None if Ua is None else _p(Ua)) != 0:
                 raise ValueError(f"orc_set_boundary({side}, {kind}) failed")
+        self._solid = None
+        if solid is not None:
+            self._solid = np.ascontiguousarray(np.asarray(solid, dtype=bool).astype(np.uint8))
+            if self._solid.size != self.Nel or lib().orc_set_solid(
+                    self.h, self._solid.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+                raise ValueError("orc_set_solid failed (mask length, or a periodic axis)")
 
     def __del__(self):
         if getattr(self, "h", None):

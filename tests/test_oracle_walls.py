@@ -41,12 +41,12 @@ def _case(p, nx, ny, x0=0.0, x1=1.0, periodic_x=False, periodic_y=False, f=_smoo
     return st
 
 
-def _oracle(st, capturing="null", flux="rusanov", sides=None):
+def _oracle(st, capturing="null", flux="rusanov", sides=None, solid=None):
     m = st.mesh
     bx = "periodic" if isinstance(m.left_bc, PeriodicBC) else "transmissive"
     by = "periodic" if isinstance(m.bottom_bc, PeriodicBC) else "transmissive"
     return Oracle(st.p, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(st.ops), bc_x=bx, bc_y=by,
-                  flux=flux, capturing=capturing, sides=sides)
+                  flux=flux, capturing=capturing, sides=sides, solid=solid)
 
 
 @pytest.mark.parametrize("flux", ["rusanov", "hllc"])
@@ -135,3 +135,46 @@ def test_reflecting_rest_state_is_steady():
     st = _case(3, 6, 5, f=lambda x, y: primitives_to_conserved(EQ, 1.0 + 0 * x, 0 * x, 0 * x, 1.0 + 0 * x))
     du = _oracle(st, sides=WALLS).residual(st.u)
     assert np.max(np.abs(du)) < 1e-12
+
+
+def _mask(nx, ny, cells):
+    m = np.zeros(nx * ny, dtype=bool)
+    for jx, jy in cells:
+        m[jy * nx + jx] = True
+    return m
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+def test_solid_column_is_a_reflecting_wall(flux):
+    """A column of solid elements at jx = 0 of [0,1] (8 columns) leaves the fluid of [1/8, 1] with
+    a wall at x = 1/8: the residual equals the 7-column mesh's with a reflecting left side (the
+    vertices are the same binary fractions), and solid elements get du = 0."""
+    p, ny = 3, 5
+    full = _case(p, 8, ny)
+    cut = _case(p, 7, ny, x0=0.125, x1=1.0)
+    mask = _mask(8, ny, [(0, jy) for jy in range(ny)])
+    du_full = _oracle(full, flux=flux, sides={"left": "reflecting"}, solid=mask).residual(full.u)
+    du_cut = _oracle(cut, flux=flux, sides={"left": "reflecting"}).residual(cut.u)
+    fluid = [jy * 8 + jx for jy in range(ny) for jx in range(1, 8)]
+    assert np.all(du_full[:, :, mask, :] == 0.0)
+    err = np.max(np.abs(du_full[:, :, fluid, :] - du_cut)) / np.max(np.abs(du_cut))
+    assert err <= 1e-13, err
+
+
+def test_solid_block_in_a_closed_box_conserves_mass():
+    p, nx, ny = 2, 10, 8
+    st = _case(p, nx, ny)
+    mask = _mask(nx, ny, [(jx, jy) for jx in (4, 5) for jy in (0, 1, 2)])   # a step on the floor
+    du = _oracle(st, sides=WALLS, solid=mask).residual(st.u)
+    assert np.all(du[:, :, mask, :] == 0.0)
+    w = st.ops.w
+    W = w[:, None, None] * w[None, :, None] * np.abs(st.metrics.J)
+    for c in (0, 3):
+        tot = np.sum(W * du[:, :, :, c])
+        assert abs(tot) <= 1e-13 * np.sum(W * np.abs(du[:, :, :, c]))
+
+
+def test_solid_mask_rejects_periodic_axes():
+    st = _case(2, 4, 4, periodic_x=True)
+    with pytest.raises(ValueError):
+        _oracle(st, solid=_mask(4, 4, [(1, 1)]))

@@ -38,6 +38,7 @@ struct frf_ctx {
     DevCtl* dCtl = nullptr;
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
     frf::MlpPacked* dMlp = nullptr;    // learned sensor weights, when set
+    unsigned char* dSolid = nullptr;   // solid-element mask (Mesh2D.solid), when set
     double side_state[4][4] = {};      // Dirichlet ghost states (left, right, bottom, top)
     bool side_state_set[4] = {false, false, false, false};
     HostOps ops{};
@@ -149,6 +150,7 @@ extern "C" int32_t frf_destroy(frf_ctx* c) {
     if (c->dRtabF) (void)hipFree(c->dRtabF);
     if (c->dCtl) (void)hipFree(c->dCtl);
     if (c->dMlp) (void)hipFree(c->dMlp);
+    if (c->dSolid) (void)hipFree(c->dSolid);
     if (c->hCtl) (void)hipHostFree(c->hCtl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -260,6 +262,30 @@ extern "C" int32_t frf_set_boundary_state(frf_ctx* c, int32_t side, const double
         c->side_state[side][q] = U[q];
     }
     c->side_state_set[side] = true;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf_set_solid_mask(frf_ctx* c, const uint8_t* mask, int64_t nel) {
+    if (!c) return FRF_E_ARG;
+    if (!mask) {   // no solid elements
+        if (c->dSolid) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->dSolid));
+            c->dSolid = nullptr;
+        }
+        return FRF_OK;
+    }
+    if (nel != (int64_t)c->nel)
+        return fail(c, FRF_E_ARG, "solid mask length must be n_elements=%d (got %lld)", c->nel, (long long)nel);
+    // Across a periodic seam the reference leaves the face of a solid neighbour unwritten
+    // (Residual2D.jl:214-225), which no kernel here reproduces: masks need walled axes.
+    if (c->pb.bc_left == FRF_BC_PERIODIC || c->pb.bc_bottom == FRF_BC_PERIODIC)
+        return fail(c, FRF_E_UNSUPPORTED, "solid masks need non-periodic sides on both axes");
+    std::vector<unsigned char> m((size_t)nel);
+    for (int64_t e = 0; e < nel; ++e) m[(size_t)e] = mask[e] ? 1 : 0;
+    if (!c->dSolid) HIPCHK(c, hipMalloc(&c->dSolid, (size_t)nel));
+    HIPCHK(c, hipMemcpyAsync(c->dSolid, m.data(), (size_t)nel, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return FRF_OK;
 }
 
@@ -434,6 +460,7 @@ static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double
     P.want_lambda = want_lambda;
     P.p = c->pb.p;
     P.mlp = c->dMlp;
+    P.solid = c->dSolid;
     const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
     for (int k = 0; k < 4; ++k) {
         P.side_bc[k] = sides[k];
@@ -461,6 +488,7 @@ static frf::ElemParams elem_params(frf_ctx* c, const double* u, int want_sigma, 
     E.p = c->pb.p;
     E.want_sigma = want_sigma;
     E.mlp = c->dMlp;
+    E.solid = c->dSolid;
     return E;
 }
 
@@ -498,6 +526,7 @@ extern "C" int32_t frf_sense(frf_ctx* c, double* sigma, double* eps) {
     if (r) return r;
     if ((r = settle(c))) return r;
     frf::ElemParams E = elem_params(c, c->dU[0], 1, c->dEps[0]);
+    E.solid = nullptr;   // sense! itself knows no solid elements; residual! zeroes their sigma
     HIPCHK(c, frf::launch_elem(c->np, c->pb.precision, E, c->ops, 0, 0, c->stream));
     if (sigma) HIPCHK(c, hipMemcpyAsync(sigma, c->dSigma, c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (eps) HIPCHK(c, hipMemcpyAsync(eps, c->dEps[0], c->nel * sizeof(double), hipMemcpyDeviceToHost, c->stream));

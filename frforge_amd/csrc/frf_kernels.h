@@ -99,6 +99,7 @@ struct StageParams {
     const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3, mixed form), or null
     int side_bc[4];       // kind of the left, right, bottom, top sides (FRF_BC_*), non-periodic axes
     double side_state[4][4];   // Dirichlet ghost states
+    const unsigned char* solid;   // solid-element mask (1 = inactive), non-periodic axes only, or null
 };
 
 struct ElemParams {
@@ -113,6 +114,7 @@ struct ElemParams {
     int p;
     int want_sigma;       // compute sigma/eps (capturing) in addition to the wave speed
     const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3), or null
+    const unsigned char* solid;   // solid-element mask: sigma = eps = 0 there, or null
 };
 
 }  // namespace frf

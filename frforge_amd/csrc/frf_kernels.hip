@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(EPB* NP) elem_kernel(ElemParams P, OpsT<NP, do
             const double dx = P.ctab[(size_t)jx * CT + L::DELTA];
             const double dy = P.rtab[(size_t)jy * CT + L::DELTA];
             const double h = dx < dy ? dx : dy;
-            const double sigma = double(se);
+            // solid elements: sigma = 0 before the viscosities (Residual2D.jl:402-405)
+            const double sigma = is_solid(P.solid, e) ? 0.0 : double(se);
             const int pp = P.p > 1 ? P.p : 1;
             double ev = 0.0;
             if (sigma > 0.0) ev = P.c_av * sigma * (h / double(pp)) * lm;

@@ -291,11 +291,24 @@ __device__ __forceinline__ void sweep_mf(const LineMats<NP>& A, const float u[NP
 }
 
 // Face flux H^ = F^ - g^ between the minus (west/south) trace um and the plus trace up.
+// wall (solid masks, Residual2D.jl:173-213,255-295): 0 an ordinary face; 1 the plus element is
+// solid, a reflecting wall for the minus element; 2 the minus element is solid, a wall for the
+// plus element.  The slot takes the flux of the fluid side; g^ is the same either way.
 template <int FLUX, int DIR, bool BR0>
 __device__ __forceinline__ void face_h(const float um[kNeq], const float up[kNeq],
                                        const float gm[kNeq], const float gp[kNeq], float tau,
-                                       float gamma, float out[kNeq]) {
-    iflux_axis<FLUX, DIR>(um, up, gamma, out);
+                                       float gamma, float out[kNeq], int wall = 0) {
+    float a[kNeq], b[kNeq];
+#pragma unroll
+    for (int c = 0; c < kNeq; ++c) {
+        a[c] = um[c];
+        b[c] = up[c];
+    }
+    if (wall == 1)
+        exterior_state<float>(kBcReflecting, um, DIR == 0 ? 1.0f : 0.0f, DIR == 0 ? 0.0f : 1.0f, nullptr, b);
+    else if (wall == 2)
+        exterior_state<float>(kBcReflecting, up, DIR == 0 ? -1.0f : 0.0f, DIR == 0 ? 0.0f : -1.0f, nullptr, a);
+    iflux_axis<FLUX, DIR>(a, b, gamma, out);
     if constexpr (BR0) {
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) out[c] -= 0.5f * (gm[c] + gp[c]) - tau * (up[c] - um[c]);
@@ -311,7 +324,7 @@ __device__ __forceinline__ void face_h(const float um[kNeq], const float up[kNeq
 template <int FLUX, int DIR, bool BR0, bool PLUS>
 __device__ __forceinline__ void boundary_h(int kind, const double* UD, const float ub[kNeq],
                                            const float gb[kNeq], float gamma, float out[kNeq]) {
-    if (kind == 2 || kind == 3) {
+    if (kind == kBcReflecting || kind == kBcDirichlet) {
         float ug[kNeq];
         const float sgn = PLUS ? -1.0f : 1.0f;   // outward normal along DIR
         exterior_state<float>(kind, ub, DIR == 0 ? sgn : 0.0f, DIR == 0 ? 0.0f : sgn, UD, ug);
@@ -568,11 +581,15 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
 #pragma unroll
             for (int i = 0; i < NP; ++i) row[c * NPP + i] = -iJy * row[c * NPP + i] - iJx * rowV[i][c];
     }
+    const unsigned char* const smask = P.solid;   // solid-element mask, or null
+    const bool solO = valid && is_solid(smask, e);
     float hE[kNeq], hN[kNeq];
     if (valid) {
         {   // E face of row j = t
             const bool in_tile = lx + 1 < nvx;
             if (in_tile || jxE >= 0) {
+                const int jxn = in_tile ? jx + 1 : jxE;
+                const bool solN = is_solid(smask, jy * nx + jxn);
                 const float* su = in_tile ? &sXW[(le + 1) * NP + t] : &sHE[ly * NP + t];
                 const int cs = in_tile ? FS : HX;
                 float up[kNeq], gp[kNeq];
@@ -584,10 +601,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) gp[c] = sg[c * cs];
                     const float epsn = in_tile ? sEps[le + 1] : sEpsH[TY + ly];
-                    const int jxn = in_tile ? jx + 1 : jxE;
                     tau = br0_tau_f(eps_e, epsn, dx, ctab[(size_t)jxn * CT + L::DELTA], P.p);
                 }
-                face_h<FLUX, 0, BR0>(uE, up, gE, gp, tau, g, hE);
+                face_h<FLUX, 0, BR0>(uE, up, gE, gp, tau, g, hE, solO == solN ? 0 : (solN ? 1 : 2));
             } else {
                 boundary_h<FLUX, 0, BR0, false>(P.side_bc[1], P.side_state[1], uE, gE, g, hE);
             }
@@ -595,6 +611,8 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
         {   // N face of column i = t
             const bool in_tile = ly + 1 < nvy;
             if (in_tile || jyN >= 0) {
+                const int jyn = in_tile ? jy + 1 : jyN;
+                const bool solN = is_solid(smask, jyn * nx + jx);
                 const float* su = in_tile ? &sYS[(le + TX) * NP + t] : &sHN[lx * NP + t];
                 const int cs = in_tile ? FS : HY;
                 float up[kNeq], gp[kNeq];
@@ -606,10 +624,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) gp[c] = sg[c * cs];
                     const float epsn = in_tile ? sEps[le + TX] : sEpsH[2 * TY + TX + lx];
-                    const int jyn = in_tile ? jy + 1 : jyN;
                     tau = br0_tau_f(eps_e, epsn, dy, rtab[(size_t)jyn * CT + L::DELTA], P.p);
                 }
-                face_h<FLUX, 1, BR0>(uN, up, gN, gp, tau, g, hN);
+                face_h<FLUX, 1, BR0>(uN, up, gN, gp, tau, g, hN, solO == solN ? 0 : (solN ? 1 : 2));
             } else {
                 boundary_h<FLUX, 1, BR0, false>(P.side_bc[3], P.side_state[3], uN, gN, g, hN);
             }
@@ -637,7 +654,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                     if constexpr (BR0)
                         tau = br0_tau_f(sEpsH[r], sEps[lo], ctab[(size_t)jxW * CT + L::DELTA],
                                         ctab[(size_t)tx0 * CT + L::DELTA], P.p);
-                    face_h<FLUX, 0, BR0>(um, uo, gm, go, tau, g, fh);
+                    const int ew = (ty0 + r) * nx;
+                    const bool sm_ = is_solid(smask, ew + jxW), sp_ = is_solid(smask, ew + tx0);
+                    face_h<FLUX, 0, BR0>(um, uo, gm, go, tau, g, fh, sm_ == sp_ ? 0 : (sp_ ? 1 : 2));
                 } else {
                     boundary_h<FLUX, 0, BR0, true>(P.side_bc[0], P.side_state[0], uo, go, g, fh);
                 }
@@ -664,7 +683,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                     if constexpr (BR0)
                         tau = br0_tau_f(sEpsH[2 * TY + l], sEps[lo], rtab[(size_t)jyS * CT + L::DELTA],
                                         rtab[(size_t)ty0 * CT + L::DELTA], P.p);
-                    face_h<FLUX, 1, BR0>(um, uo, gm, go, tau, g, fh);
+                    const bool sm_ = is_solid(smask, jyS * nx + tx0 + l);
+                    const bool sp_ = is_solid(smask, ty0 * nx + tx0 + l);
+                    face_h<FLUX, 1, BR0>(um, uo, gm, go, tau, g, fh, sm_ == sp_ ? 0 : (sp_ ? 1 : 2));
                 } else {
                     boundary_h<FLUX, 1, BR0, true>(P.side_bc[2], P.side_state[2], uo, go, g, fh);
                 }
@@ -724,7 +745,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             double* po = P.uout + ((size_t)c * nel + e) * NPP + t * NP;
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
-                const float d = row[c * NPP + i] - iJx * (dW[c] * op.gL[i] + dE[c] * op.gR[i]);
+                const float d = solO ? 0.0f : row[c * NPP + i] - iJx * (dW[c] * op.gL[i] + dE[c] * op.gR[i]);
                 double un;
                 if (stage == 0) {
                     un = double(d);
@@ -823,7 +844,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             }
             const float h = dx < dy ? dx : dy;
             const int pp = P.p > 1 ? P.p : 1;
-            P.eps_out[e] = sigma > 0.0f ? double(float(P.c_av) * sigma * (h / float(pp)) * lm) : 0.0;
+            P.eps_out[e] = (sigma > 0.0f && !solO) ? double(float(P.c_av) * sigma * (h / float(pp)) * lm) : 0.0;
         }
         lm_el = lm;
     }

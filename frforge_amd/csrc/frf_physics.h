@@ -258,10 +258,12 @@ __device__ __forceinline__ void iflux_axis(const R uL[4], const R uR[4], R gamma
 // exterior_state of a non-periodic side (BoundaryConditions.jl:43-110) for the outward unit
 // normal (nx, ny): kind 2 reflects the velocity (reflect_conserved, with the floored density),
 // kind 3 takes the given state, anything else (transmissive) the interior trace.
+constexpr int kBcReflecting = 2, kBcDirichlet = 3;   // FRF_BC_REFLECTING, FRF_BC_DIRICHLET
+
 template <typename R>
 __device__ __forceinline__ void exterior_state(int kind, const R U[4], R nx, R ny, const double* UD,
                                                R ug[4]) {
-    if (kind == 2) {
+    if (kind == kBcReflecting) {
         const R rho = rmax(U[0], R(kEps64));
         const R u = dv(U[1], rho), v = dv(U[2], rho);
         const R un = u * nx + v * ny;
@@ -271,13 +273,18 @@ __device__ __forceinline__ void exterior_state(int kind, const R U[4], R nx, R n
         ug[1] = rho * ugx;
         ug[2] = rho * vgy;
         ug[3] = U[3];
-    } else if (kind == 3) {
+    } else if (kind == kBcDirichlet) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) ug[c] = R(UD[c]);
     } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) ug[c] = U[c];
     }
+}
+
+// Solid-element mask (Mesh2D.jl:31-32,93-97): true for an inactive element; a null mask has none.
+__device__ __forceinline__ bool is_solid(const unsigned char* mask, int e) {
+    return mask != nullptr && mask[e] != 0;
 }
 
 // Element wave speed term max(|u| + c, |v| + c) at one point (PerssonAV2D.jl:62-82,

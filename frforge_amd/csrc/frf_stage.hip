@@ -353,14 +353,20 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
     }
 
     // ------------------------------------------------------------------ P2: faces (once each)
+    // Solid masks (Residual2D.jl:173-213,255-295): a face between a fluid and a solid element is
+    // a reflecting wall for the fluid side, and its flux slot holds what the fluid side uses;
+    // solid elements get du = 0.  BR0 averages over such faces as over any other.
+    const unsigned char* const smask = P.solid;
+    const bool solO = valid && is_solid(smask, e);
     R fhE[kNeq], fhN[kNeq], ghE[kNeq], ghN[kNeq];
     if (valid) {
         {   // east face of row j = t, with this element's east normal
             const int j = t;
             const bool in_tile = lx + 1 < nvx;
             const bool has_n = in_tile || jxE >= 0;
+            const int jxn = in_tile ? jx + 1 : jxE;
             const R a = rtR[L::FE + j], b = ctR[L::FE + j];   // y_eta, x_eta at the E face
-            const R s = face_s(a, b);
+            R s = face_s(a, b);
             const R nxo = dv(a, s), nyo = dv(-b, s);
             R fh[4];
             if (has_n) {
@@ -368,11 +374,24 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 R upv[kNeq];
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) upv[c] = up[c];
-                iflux<FLUX>(trE, upv, nxo, nyo, g, fh);
+                const bool solN = is_solid(smask, jy * nx + jxn);
+                if (solO == solN) {
+                    iflux<FLUX>(trE, upv, nxo, nyo, g, fh);
+                } else if (solN) {   // wall on this element's east face
+                    R ug[4];
+                    exterior_state<R>(kBcReflecting, trE, nxo, nyo, nullptr, ug);
+                    iflux<FLUX>(trE, ug, nxo, nyo, g, fh);
+                } else {             // wall on the east element's west face, with its metrics
+                    const R aw = rtR[L::FW + j], bw = ctabR[(size_t)jxn * CT + L::FW + j];
+                    s = face_s(aw, bw);
+                    const R nxw = dv(-aw, s), nyw = dv(bw, s);
+                    R ug[4];
+                    exterior_state<R>(kBcReflecting, upv, nxw, nyw, nullptr, ug);
+                    iflux<FLUX>(ug, upv, -nxw, -nyw, g, fh);
+                }
                 if constexpr (BR0) {
                     const Lane<R> gp = in_tile ? Lane<R>{&sGx[(le + 1) * NP + j], FS} : Lane<R>{&sGHE[ly * NP + j], TY * NP};
                     const double epsn = in_tile ? sEpsT[le + 1] : sEpsH[TY + ly];
-                    const int jxn = in_tile ? jx + 1 : jxE;
                     const R tau = br0_tau<R>(eps_e, epsn, dx, P.ctab[(size_t)jxn * CT + L::DELTA], P.p);
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) ghE[c] = br0_avg(gxE[c], gp[c], trE[c], upv[c], tau);
@@ -391,30 +410,32 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             if (lx == 0) {   // west face of the tile's first column
                 R fw[4];
                 R sw;
+                // a solid west neighbour is a wall on this fluid element's own west face
+                const bool wallW = jxW >= 0 && !solO && is_solid(smask, jy * nx + jxW);
                 if (jxW >= 0) {
                     const R aw = rtR[L::FE + j], bw = ctabR[(size_t)jxW * CT + L::FE + j];
                     sw = face_s(aw, bw);
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHW[c * (TY * NP) + ly * NP + j];
-                    iflux<FLUX>(um, trW, dv(aw, sw), dv(-bw, sw), g, fw);
+                    if (!wallW) iflux<FLUX>(um, trW, dv(aw, sw), dv(-bw, sw), g, fw);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[ly], eps_e, P.ctab[(size_t)jxW * CT + L::DELTA], dx, P.p);
 #pragma unroll
                         for (int c = 0; c < kNeq; ++c)
                             sG0x[c * (TY * NP) + ly * NP + j] = br0_avg(sGHW[c * (TY * NP) + ly * NP + j], gxW[c], um[c], trW[c], tau);
                     }
-                } else {
+                } else if constexpr (BR0) {
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) sG0x[c * (TY * NP) + ly * NP + j] = gxW[c];
+                }
+                if (jxW < 0 || wallW) {
                     const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];   // own west face
                     sw = face_s(aw, bw);
                     const R nxw = dv(-aw, sw), nyw = dv(bw, sw);
                     R ug[4];
-                    exterior_state<R>(P.side_bc[0], trW, nxw, nyw, P.side_state[0], ug);
+                    exterior_state<R>(wallW ? kBcReflecting : P.side_bc[0], trW, nxw, nyw, P.side_state[0], ug);
                     iflux<FLUX>(ug, trW, -nxw, -nyw, g, fw);
-                    if constexpr (BR0) {
-#pragma unroll
-                        for (int c = 0; c < kNeq; ++c) sG0x[c * (TY * NP) + ly * NP + j] = gxW[c];
-                    }
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0x[c * (TY * NP) + ly * NP + j] = fw[c] * sw;
@@ -424,8 +445,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             const int i = t;
             const bool in_tile = ly + 1 < nvy;
             const bool has_n = in_tile || jyN >= 0;
+            const int jyn = in_tile ? jy + 1 : jyN;
             const R a = rtR[L::FN + i], b = ctR[L::FN + i];   // y_xi, x_xi at the N face
-            const R s = face_s(a, b);
+            R s = face_s(a, b);
             const R nxo = dv(-a, s), nyo = dv(b, s);
             R fh[4];
             if (has_n) {
@@ -433,11 +455,24 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 R upv[kNeq];
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) upv[c] = up[c];
-                iflux<FLUX>(trN, upv, nxo, nyo, g, fh);
+                const bool solN = is_solid(smask, jyn * nx + jx);
+                if (solO == solN) {
+                    iflux<FLUX>(trN, upv, nxo, nyo, g, fh);
+                } else if (solN) {   // wall on this element's north face
+                    R ug[4];
+                    exterior_state<R>(kBcReflecting, trN, nxo, nyo, nullptr, ug);
+                    iflux<FLUX>(trN, ug, nxo, nyo, g, fh);
+                } else {             // wall on the north element's south face, with its metrics
+                    const R as = rtabR[(size_t)jyn * CT + L::FS + i], bs = ctR[L::FS + i];
+                    s = face_s(as, bs);
+                    const R nxs = dv(as, s), nys = dv(-bs, s);
+                    R ug[4];
+                    exterior_state<R>(kBcReflecting, upv, nxs, nys, nullptr, ug);
+                    iflux<FLUX>(ug, upv, -nxs, -nys, g, fh);
+                }
                 if constexpr (BR0) {
                     const Lane<R> gp = in_tile ? Lane<R>{&sGy[(le + TX) * NP + i], FS} : Lane<R>{&sGHN[lx * NP + i], TX * NP};
                     const double epsn = in_tile ? sEpsT[le + TX] : sEpsH[2 * TY + TX + lx];
-                    const int jyn = in_tile ? jy + 1 : jyN;
                     const R tau = br0_tau<R>(eps_e, epsn, dy, P.rtab[(size_t)jyn * CT + L::DELTA], P.p);
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) ghN[c] = br0_avg(gyN[c], gp[c], trN[c], upv[c], tau);
@@ -456,30 +491,32 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             if (ly == 0) {   // south face of the tile's first row
                 R fs[4];
                 R ss;
+                // a solid south neighbour is a wall on this fluid element's own south face
+                const bool wallS = jyS >= 0 && !solO && is_solid(smask, jyS * nx + jx);
                 if (jyS >= 0) {
                     const R as = rtabR[(size_t)jyS * CT + L::FN + i], bs = ctR[L::FN + i];
                     ss = face_s(as, bs);
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHS[c * (TX * NP) + lx * NP + i];
-                    iflux<FLUX>(um, trS, dv(-as, ss), dv(bs, ss), g, fs);
+                    if (!wallS) iflux<FLUX>(um, trS, dv(-as, ss), dv(bs, ss), g, fs);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[2 * TY + lx], eps_e, P.rtab[(size_t)jyS * CT + L::DELTA], dy, P.p);
 #pragma unroll
                         for (int c = 0; c < kNeq; ++c)
                             sG0y[c * (TX * NP) + lx * NP + i] = br0_avg(sGHS[c * (TX * NP) + lx * NP + i], gyS[c], um[c], trS[c], tau);
                     }
-                } else {
+                } else if constexpr (BR0) {
+#pragma unroll
+                    for (int c = 0; c < kNeq; ++c) sG0y[c * (TX * NP) + lx * NP + i] = gyS[c];
+                }
+                if (jyS < 0 || wallS) {
                     const R as = rtR[L::FS + i], bs = ctR[L::FS + i];   // own south face
                     ss = face_s(as, bs);
                     const R nxs = dv(as, ss), nys = dv(-bs, ss);
                     R ug[4];
-                    exterior_state<R>(P.side_bc[2], trS, nxs, nys, P.side_state[2], ug);
+                    exterior_state<R>(wallS ? kBcReflecting : P.side_bc[2], trS, nxs, nys, P.side_state[2], ug);
                     iflux<FLUX>(ug, trS, -nxs, -nys, g, fs);
-                    if constexpr (BR0) {
-#pragma unroll
-                        for (int c = 0; c < kNeq; ++c) sG0y[c * (TX * NP) + lx * NP + i] = gyS[c];
-                    }
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0y[c * (TX * NP) + lx * NP + i] = fs[c] * ss;
@@ -566,7 +603,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             for (int c = 0; c < kNeq; ++c) {
                 R d = rowv[i][c] - (fW[c] * op.gL[i] + fE[c] * op.gR[i]) * iJ - myB[c * NPP + j * NP + i];
                 if constexpr (BR0) d += (hW[c] * op.gL[i] + hE[c] * op.gR[i]) * iJx;
-                du[i][c] = d;
+                du[i][c] = solO ? R(0) : d;
             }
         }
         const int stage = P.stage;
@@ -663,7 +700,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             const double sigma = double(persson_sigma(ehigh, etot, P.s0, P.kappa, P.eps_floor));
             const double h = dx < dy ? dx : dy;
             const int pp = P.p > 1 ? P.p : 1;
-            P.eps_out[e] = sigma > 0.0 ? P.c_av * sigma * (h / double(pp)) * lm : 0.0;
+            P.eps_out[e] = (sigma > 0.0 && !solO) ? P.c_av * sigma * (h / double(pp)) * lm : 0.0;
         }
         sRed[le * NP] = lm;
     }

@@ -24,7 +24,7 @@ STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
 EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_boundary_state",
-            "frf_set_learned_sensor",
+            "frf_set_solid_mask", "frf_set_learned_sensor",
             "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     L.frf_set_geometry.argtypes = [vp, dp, dp]
     L.frf_set_learned_sensor.argtypes = [vp, C.c_void_p, C.c_int64]
     L.frf_set_boundary_state.argtypes = [vp, C.c_int32, dp]
+    L.frf_set_solid_mask.argtypes = [vp, C.c_void_p, C.c_int64]
     L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf_residual.argtypes = [vp, dp, C.c_int64]
@@ -208,6 +209,15 @@ class DeviceContext:
         if Ua.shape != (4,):
             raise ValueError("a Dirichlet state has 4 conserved components")
         self._chk(self.lib.frf_set_boundary_state(self.h, SIDES[side], _dp(Ua)), "frf_set_boundary_state")
+
+    def set_solid_mask(self, mask):
+        """Solid-element mask (Mesh2D.solid: Nel booleans, element e = jx + nx*jy), or None."""
+        if mask is None:
+            self._chk(self.lib.frf_set_solid_mask(self.h, None, 0), "frf_set_solid_mask")
+            return
+        m = np.ascontiguousarray(np.asarray(mask, dtype=bool).ravel(), dtype=np.uint8)
+        self._chk(self.lib.frf_set_solid_mask(self.h, m.ctypes.data_as(C.c_void_p), m.size),
+                  "frf_set_solid_mask")
 
     def set_learned_sensor(self, packed):
         """Weights of the learned sensor (bytes from MLPWeights.pack), or None for Persson."""

@@ -7,6 +7,7 @@ reference is 1-based, Mesh2D.jl:100).
 from __future__ import annotations
 
 from dataclasses import dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -79,6 +80,8 @@ class Mesh2D:
     right_bc: AbstractBC = field(default_factory=PeriodicBC)
     bottom_bc: AbstractBC = field(default_factory=PeriodicBC)
     top_bc: AbstractBC = field(default_factory=PeriodicBC)
+    # optional solid-element mask, True = inactive / wall interior (Mesh2D.jl:31-32,65-69)
+    solid: Optional[np.ndarray] = None
 
     def __post_init__(self):
         if self.nx < 1 or self.ny < 1:
@@ -91,6 +94,11 @@ class Mesh2D:
             raise ValueError("x-periodic requires both left and right PeriodicBC")
         if isinstance(self.bottom_bc, PeriodicBC) != isinstance(self.top_bc, PeriodicBC):
             raise ValueError("y-periodic requires both bottom and top PeriodicBC")
+        if self.solid is not None:
+            m = np.asarray(self.solid, dtype=bool).ravel()
+            if m.size != self.nx * self.ny:
+                raise ValueError(f"solid mask length must be n_elements={self.nx * self.ny}")
+            self.solid = m
         self.x_vertices = _linspace(float(self.x_left), float(self.x_right), self.nx)
         self.y_vertices = _linspace(float(self.y_bottom), float(self.y_top), self.ny)
         self.dx = np.diff(self.x_vertices)
@@ -104,6 +112,10 @@ class Mesh2D:
 
     def element_index(self, jx: int, jy: int) -> int:
         return jy * self.nx + jx
+
+    def is_solid(self, e: int) -> bool:
+        """True if element ``e`` is marked solid (Mesh2D.jl:93-97)."""
+        return self.solid is not None and bool(self.solid[e])
 
     def element_coords(self, e: int):
         jy, jx = divmod(e, self.nx)

@@ -59,7 +59,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
            _bc_name(mesh.top_bc), state.scheme.flux, state.precision, kind,
            tuple(sorted(prm.items())), float(eq.gamma), id(state.ops),
            getattr(method, "weights_path", None),
-           tuple((k, b.state) for k, b in _sides(mesh).items() if isinstance(b, DirichletBC)))
+           tuple((k, b.state) for k, b in _sides(mesh).items() if isinstance(b, DirichletBC)),
+           None if mesh.solid is None else mesh.solid.tobytes())
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
         return dev
@@ -75,6 +76,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
     for side, bc in _sides(mesh).items():
         if isinstance(bc, DirichletBC):
             dev.set_boundary_state(side, bc.state)
+    if mesh.solid is not None:
+        dev.set_solid_mask(mesh.solid)
     if getattr(method, "weights_path", None) is not None:   # learned sensor (frforge_amd.learned)
         dev.set_learned_sensor(method.weights().pack())
     dev.key = key

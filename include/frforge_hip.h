@@ -86,6 +86,13 @@ int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y
  * every Dirichlet side. */
 int32_t frf_set_boundary_state(frf_ctx* ctx, int32_t side, const double* U);
 
+/* Solid-element mask (Mesh2D(...; solid = mask), Mesh2D.jl:31-32,46,65-69): nel bytes in element
+ * order e = jx + nx*jy, nonzero = solid.  residual! (Residual2D.jl:140-416) then treats a face
+ * between a fluid and a solid element as a reflecting wall for the fluid side and gives solid
+ * elements sigma = 0 and du = 0; BR0 ignores the mask.  Both axes must be non-periodic
+ * (FRF_E_UNSUPPORTED otherwise).  NULL removes the mask. */
+int32_t frf_set_solid_mask(frf_ctx* ctx, const uint8_t* mask, int64_t nel);
+
 /* Learned troubled-cell sensor in place of the Persson sigma, for the sense! hook of a Persson
  * AV method (Interfaces.jl:79-87, PerssonAV2D.jl:20-59; the viscosity formula of
  * PerssonAV2D.jl:89-109 is unchanged): an MLP 16 -> 32 -> 32 -> 1 on the 16 Legendre modes of

@@ -8,6 +8,7 @@ from frforge_amd import (Euler2D, Mesh2D, NullCapturing, PerssonAVMethod, Period
                          SchemeConfig, TransmissiveBC, allocate_state, build_operators,
                          set_initial_condition)
 from frforge_amd.cases import riemann2d_ic, seeded_quadrant_states, vortex_ic
+from frforge_amd.mesh import DirichletBC, ReflectingBC
 from oracle.oracle import Oracle
 
 EQ = Euler2D(1.4)
@@ -61,14 +62,27 @@ def rel_err_per_component(a, b):
     return max(out)
 
 
+def _side_spec(bc):
+    if isinstance(bc, ReflectingBC):
+        return "reflecting"
+    if isinstance(bc, DirichletBC):
+        return ("dirichlet", bc.state)
+    return "transmissive"
+
+
 def oracle_for_state(state, method, eq=EQ):
-    """The oracle problem matching a 2D state (mesh, BCs, flux) and a capturing method."""
+    """The oracle problem matching a 2D state (mesh, sides, solid mask, flux) and a capturing
+    method."""
     kind, prm = method.device_capturing()
-    bx = "periodic" if isinstance(state.mesh.left_bc, PeriodicBC) else "transmissive"
-    by = "periodic" if isinstance(state.mesh.bottom_bc, PeriodicBC) else "transmissive"
     m = state.mesh
+    bx = "periodic" if isinstance(m.left_bc, PeriodicBC) else "transmissive"
+    by = "periodic" if isinstance(m.bottom_bc, PeriodicBC) else "transmissive"
+    sides = {k: _side_spec(b) for k, b in (("left", m.left_bc), ("right", m.right_bc),
+                                            ("bottom", m.bottom_bc), ("top", m.top_bc))
+             if not isinstance(b, PeriodicBC)}
     return Oracle(state.p, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(state.ops), bc_x=bx,
-                  bc_y=by, flux=state.scheme.flux, capturing=kind, gamma=eq.gamma, **prm)
+                  bc_y=by, flux=state.scheme.flux, capturing=kind, gamma=eq.gamma, sides=sides,
+                  solid=m.solid, **prm)
 
 
 def oracle_integrator_2d(state, eq, method, t_final, cfl, dt=None):

@@ -1,0 +1,48 @@
+"""Host side of the optional wall cases: the forward-facing-step mask, the Mesh2D solid keyword,
+and the FFS case runner driven through the oracle's time loop (no GPU)."""
+import numpy as np
+import pytest
+
+from frforge_amd import Mesh2D
+from frforge_amd.cases2d_optional import make_ffs_solid_mask, run_forward_facing_step
+
+from helpers import oracle_integrator_2d
+
+
+def test_ffs_mask_default_case():
+    """24 x 8 elements on 1.5 x 0.5 with the step at x >= 0.3, y <= 0.1: element centres
+    (jx + 1/2) / 16 and (jy + 1/2) / 16 put columns 5..23 of rows 0 and 1 inside."""
+    m = make_ffs_solid_mask(24, 8, 1.5, 0.5, step_x=0.3, step_h=0.1).reshape(8, 24)
+    want = np.zeros((8, 24), dtype=bool)
+    want[0:2, 5:] = True
+    assert np.array_equal(m, want)
+
+
+def test_ffs_mask_boundary_centres_are_inside():
+    """A centre exactly on the step edge counts as solid (>= and <=): 8 x 8 on 1 x 1, centres
+    (j + 1/2) / 8 in binary, step_x = 4.5 / 8 and step_h = 1.5 / 8."""
+    m = make_ffs_solid_mask(8, 8, 1.0, 1.0, step_x=0.5625, step_h=0.1875).reshape(8, 8)
+    assert m[1, 4] and m[0, 7] and not m[2, 4] and not m[1, 3]
+    assert m.sum() == 2 * 4
+
+
+def test_mesh_solid_length_checked():
+    with pytest.raises(ValueError, match="n_elements=12"):
+        Mesh2D(0.0, 1.0, 0.0, 1.0, 4, 3, solid=np.zeros(11, dtype=bool))
+    mesh = Mesh2D(0.0, 1.0, 0.0, 1.0, 4, 3, solid=np.eye(3, 4, dtype=bool))
+    assert mesh.is_solid(0) and mesh.is_solid(5) and not mesh.is_solid(1)
+
+
+def test_ffs_case_through_the_oracle():
+    """The reference's reduced FFS of its optional suite (p = 1, 12 x 4, t = 0.02,
+    Cases2D_Optional.jl:302-315) runs to the end without divergence; solid elements keep the
+    inflow state."""
+    case, st, _ = run_forward_facing_step(p=1, nx=12, ny=4, t_final=0.02, cfl=0.025, Lx=1.0,
+                                          Ly=0.4, step_x=0.3, step_h=0.1, M_in=2.0,
+                                          integrator=oracle_integrator_2d)
+    assert case["pass"] and not case["diverged"] and not case["nan_detected"]
+    assert case["metrics"]["n_solid"] == 8 and case["metrics"]["n_steps"] > 0
+    assert case["name"] == "forward_facing_step_p1_persson_av"
+    solid = st.mesh.solid
+    assert np.all(st.u[:, :, solid, :] == st.u[0, 0, solid, :][0])
+    assert np.all(np.isfinite(st.u))

@@ -170,7 +170,11 @@ struct orc_ctx {
     orc_problem pb;
     int bc_kind[4];                   /* per side: left, right, bottom, top */
     unsigned char* solid;             /* solid-element mask (Mesh2D.solid), or NULL */
-    double bc_state[4][4];            /* Dirichlet ghost states */
+    double bc_state[4][4];            /* Dirichlet ghost states; split sides: the state below */
+    double split[4][4];               /* split sides: switch point a + (b + c t) / d */
+    int split_hi[4];                  /* split sides: kind above the switch point */
+    double split_state[4][4];         /* ... and its Dirichlet state */
+    double t_res;                     /* time of the residual being evaluated */
     int Np, Nel;
     double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
     double *LU; int* piv;             /* LU of V with partial pivoting (sensor solves) */
@@ -483,12 +487,43 @@ int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U) {
     return 0;
 }
 
-/* exterior_state (BoundaryConditions.jl:64-110): transmissive copies the interior trace,
+/* A GhostStateBC whose ghost switches at a moving point along the side (BoundaryConditions.jl:
+ * 112-130, in the form the reduced double-Mach reflection uses, Cases2D_Optional.jl:72-84):
+ * at a face point with coordinate s along the side (x on the bottom and top, y on the left and
+ * right) and time t, the ghost is U_lo where s < a + (b + c t) / d, and otherwise that of kind_hi
+ * (reflecting, or Dirichlet with U_hi). */
+int orc_set_boundary_split(orc_ctx* h, int side, const double* coef, const double* U_lo,
+                           int kind_hi, const double* U_hi) {
+    if (side < 0 || side > 3 || !coef || !U_lo) return -1;
+    int axis_bc = side < 2 ? h->pb.bc_x : h->pb.bc_y;
+    if (axis_bc == ORC_BC_PERIODIC) return -1;
+    if (kind_hi != ORC_BC_REFLECTING && kind_hi != ORC_BC_DIRICHLET) return -1;
+    if (kind_hi == ORC_BC_DIRICHLET && !U_hi) return -1;
+    h->bc_kind[side] = ORC_BC_SPLIT;
+    memcpy(h->split[side], coef, sizeof h->split[side]);
+    memcpy(h->bc_state[side], U_lo, sizeof h->bc_state[side]);
+    h->split_hi[side] = kind_hi;
+    if (kind_hi == ORC_BC_DIRICHLET) memcpy(h->split_state[side], U_hi, sizeof h->split_state[side]);
+    return 0;
+}
+
+/* exterior_state (BoundaryConditions.jl:64-130): transmissive copies the interior trace,
  * reflecting mirrors the velocity through the outward unit normal with the floored density
- * (reflect_conserved, :43-60), Dirichlet takes the given state. */
+ * (reflect_conserved, :43-60), Dirichlet takes the given state; a split side picks one of two
+ * at the face point's coordinate s along the side. */
 static void exterior_state(const orc_ctx* h, int side, const double* U, double nxo, double nyo,
-                           double* ug) {
+                           double s, double* ug) {
     int kind = side < 0 ? ORC_BC_REFLECTING : h->bc_kind[side];   /* side -1: solid wall */
+    const double* UD = side < 0 ? NULL : h->bc_state[side];
+    if (kind == ORC_BC_SPLIT) {
+        const double* k = h->split[side];
+        if (!(s < k[0] + (k[1] + k[2] * h->t_res) / k[3])) {
+            kind = h->split_hi[side];
+            UD = h->split_state[side];
+        } else {
+            kind = ORC_BC_DIRICHLET;
+        }
+    }
     if (kind == ORC_BC_REFLECTING) {
         double rho = jmax(U[0], EPS);
         double u = U[1] / rho;
@@ -501,7 +536,7 @@ static void exterior_state(const orc_ctx* h, int side, const double* U, double n
         ug[2] = rho * vgy;
         ug[3] = U[3];
     } else if (kind == ORC_BC_DIRICHLET) {
-        memcpy(ug, h->bc_state[side], 4 * sizeof(double));
+        memcpy(ug, UD, 4 * sizeof(double));
     } else {
         memcpy(ug, U, 4 * sizeof(double));
     }
@@ -513,7 +548,13 @@ static void boundary_fhat(const orc_ctx* h, double* fhat, int q, int e, const do
                           double nxo, double nyo, double sJ, int into_domain, int side) {
     int Np = h->Np;
     double ug[4], fh[4];
-    exterior_state(h, side, uint_, nxo, nyo, ug);
+    double X = 0.0, Y = 0.0;   /* the face point, physical_xy(mesh, e, +-1, xi_q) */
+    if (side == ORC_SIDE_LEFT || side == ORC_SIDE_RIGHT)
+        physical_xy(h, e, side == ORC_SIDE_LEFT ? -1.0 : 1.0, h->xi[q], &X, &Y);
+    else if (side >= 0)
+        physical_xy(h, e, h->xi[q], side == ORC_SIDE_BOTTOM ? -1.0 : 1.0, &X, &Y);
+    const double s = (side == ORC_SIDE_LEFT || side == ORC_SIDE_RIGHT) ? Y : X;
+    exterior_state(h, side, uint_, nxo, nyo, s, ug);
     if (into_domain) iflux(h, ug, uint_, -nxo, -nyo, fh);
     else iflux(h, uint_, ug, nxo, nyo, fh);
     for (int c = 0; c < 4; ++c) fhat[FIDX(q, c, e)] = fh[c] * sJ;
@@ -528,7 +569,7 @@ static void apply_br0(orc_ctx* h, const double* u, const double* eps, double* du
 static void apply_dd(orc_ctx* h, const double* u, const double* eps, double* du);
 
 void orc_residual(orc_ctx* h, const double* u, double t, double* du) {
-    (void)t;
+    h->t_res = t;   /* the boundary time (Residual2D.jl:143) */
     const int Np = h->Np, Nel = h->Nel, nx = h->pb.nx, ny = h->pb.ny;
     const double *D = h->D, *gLv = h->gLxi, *gRv = h->gRxi, *lL = h->lL, *lR = h->lR;
     const double g = h->pb.gamma;

@@ -22,7 +22,8 @@
 extern "C" {
 #endif
 
-enum { ORC_BC_PERIODIC = 0, ORC_BC_TRANSMISSIVE = 1, ORC_BC_REFLECTING = 2, ORC_BC_DIRICHLET = 3 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_TRANSMISSIVE = 1, ORC_BC_REFLECTING = 2, ORC_BC_DIRICHLET = 3,
+       ORC_BC_SPLIT = 4 };
 enum { ORC_SIDE_LEFT = 0, ORC_SIDE_RIGHT = 1, ORC_SIDE_BOTTOM = 2, ORC_SIDE_TOP = 3 };
 enum { ORC_FLUX_RUSANOV = 0, ORC_FLUX_HLLC = 1 };
 enum { ORC_CAPT_NULL = 0, ORC_CAPT_PERSSON_BR0 = 1, ORC_CAPT_PERSSON_DD = 2 };
@@ -63,6 +64,13 @@ const double* orc_metric(orc_ctx* h, int which);
  * (exterior_state, BoundaryConditions.jl:64-110).  Returns 0, or -1 for a periodic axis or an
  * unknown kind. */
 int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U);
+
+/* A ghost state that switches at a moving point along a non-periodic side (GhostStateBC,
+ * BoundaryConditions.jl:112-130, as Cases2D_Optional.jl:72-84 uses it): at a face point with
+ * coordinate s along the side and time t, U_lo where s < coef[0] + (coef[1] + coef[2] t) /
+ * coef[3], otherwise kind_hi (ORC_BC_REFLECTING, or ORC_BC_DIRICHLET with U_hi). */
+int orc_set_boundary_split(orc_ctx* h, int side, const double* coef, const double* U_lo,
+                           int kind_hi, const double* U_hi);
 /* Solid-element mask (Mesh2D.solid, Mesh2D.jl:31-95): faces between a fluid and a solid element
  * are reflecting walls of the fluid element, solid elements get sigma = 0 and du = 0
  * (Residual2D.jl:140-416).  mask[Nel] nonzero = solid; NULL clears it.  Non-periodic axes only

@@ -64,6 +64,8 @@ def lib():
         L.orc_set_solid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.orc_set_boundary.restype = C.c_int
         L.orc_set_boundary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_set_boundary_split.restype = C.c_int
+        L.orc_set_boundary_split.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, dp]
         L.orc_sense.argtypes = [C.c_void_p, dp, dp]
         L.orc_element_viscosities.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_apply_br0_eps.argtypes = [C.c_void_p, dp, dp, dp]
@@ -136,7 +138,8 @@ class Oracle:
                  flux="rusanov", capturing="null", gamma=1.4, kappa=4.0, s0_factor=-4.0,
                  eps_floor=1e-16, c_av=0.1, sides=None, solid=None):
         """sides: optional {"left"|"right"|"bottom"|"top": "transmissive" | "reflecting" |
-        ("dirichlet", U[4])} for the non-periodic axes."""
+        ("dirichlet", U[4]) | ("split", (a, b, c, d), U_lo, "reflecting" | "dirichlet", U_hi)}
+        for the non-periodic axes (split: U_lo where s < a + (b + c t) / d along the side)."""
         pb = OrcProblem(p, nx, ny, BC_PERIODIC if bc_x == "periodic" else BC_TRANSMISSIVE,
                         BC_PERIODIC if bc_y == "periodic" else BC_TRANSMISSIVE, FLUX[flux],
                         CAPT[capturing], gamma, kappa, s0_factor, eps_floor, c_av)
@@ -153,6 +156,16 @@ class Oracle:
             raise ValueError("orc_create failed")
         self._bc_states = []
         for side, spec in (sides or {}).items():
+            if not isinstance(spec, str) and spec[0] == "split":
+                _, coef, U_lo, kind_hi, U_hi = spec
+                arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (coef, U_lo)]
+                hi = None if U_hi is None else np.ascontiguousarray(U_hi, dtype=np.float64)
+                self._bc_states += arrs + [hi]
+                if lib().orc_set_boundary_split(self.h, self.SIDE[side], _p(arrs[0]), _p(arrs[1]),
+                                                self.BC_KIND[kind_hi],
+                                                None if hi is None else _p(hi)) != 0:
+                    raise ValueError(f"orc_set_boundary_split({side}) failed")
+                continue
             kind, U = (spec, None) if isinstance(spec, str) else spec
             Ua = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
             self._bc_states.append(Ua)

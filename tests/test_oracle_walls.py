@@ -178,3 +178,85 @@ def test_solid_mask_rejects_periodic_axes():
     st = _case(2, 4, 4, periodic_x=True)
     with pytest.raises(ValueError):
         _oracle(st, solid=_mask(4, 4, [(1, 1)]))
+
+
+# --- split ghost sides (GhostStateBC with a moving switch point, Cases2D_Optional.jl:72-84) ---
+
+U_LO = primitives_to_conserved(EQ, 1.7, 0.4, -0.2, 1.9)
+U_HI = primitives_to_conserved(EQ, 0.8, -0.1, 0.05, 0.7)
+
+
+def _split(coef, hi="reflecting"):
+    return ("split", coef, U_LO, hi, U_HI if hi == "dirichlet" else None)
+
+
+@pytest.mark.parametrize("side", ["left", "right", "bottom", "top"])
+@pytest.mark.parametrize("hi", ["reflecting", "dirichlet"])
+def test_split_side_far_switch_points_reduce_to_constant_sides(side, hi):
+    """A switch point below every face point gives the hi kind everywhere, one above every face
+    point the low Dirichlet state: the residuals equal those of the constant sides bit for bit."""
+    st = _case(3, 8, 6)
+    u = st.u
+    below = _oracle(st, "persson_br0", "hllc", {side: _split((-1e9, 0.0, 0.0, 1.0), hi)}).residual(u)
+    above = _oracle(st, "persson_br0", "hllc", {side: _split((1e9, 0.0, 0.0, 1.0), hi)}).residual(u)
+    hi_spec = "reflecting" if hi == "reflecting" else ("dirichlet", U_HI)
+    want_hi = _oracle(st, "persson_br0", "hllc", {side: hi_spec}).residual(u)
+    want_lo = _oracle(st, "persson_br0", "hllc", {side: ("dirichlet", U_LO)}).residual(u)
+    assert np.array_equal(below, want_hi)
+    assert np.array_equal(above, want_lo)
+
+
+def _split_at_edge(st, side, hi, coef, t, k):
+    """Residual with the switch point at vertex k of the side's axis (time t), and the element
+    by element composition of the two constant sides that it must equal: the Gauss-Legendre face
+    points lie strictly inside their elements, so each element sees one of the two states."""
+    m = st.mesh
+    got = _oracle(st, "null", "rusanov", {side: _split(coef, hi)}).residual(st.u, t)
+    hi_spec = "reflecting" if hi == "reflecting" else ("dirichlet", U_HI)
+    r_hi = _oracle(st, "null", "rusanov", {side: hi_spec}).residual(st.u, t)
+    r_lo = _oracle(st, "null", "rusanov", {side: ("dirichlet", U_LO)}).residual(st.u, t)
+    e = np.arange(m.n_elements)
+    along = (e % m.nx) if side in ("bottom", "top") else (e // m.nx)
+    want = np.where((along < k)[None, None, :, None], r_lo, r_hi)
+    return got, want, r_lo, r_hi
+
+
+@pytest.mark.parametrize("side", ["bottom", "top", "left", "right"])
+def test_split_side_switch_at_an_element_edge(side):
+    """8 x 8 elements on the unit square (binary vertices): with the switch point at x (or y) =
+    3/8 the elements left of it take the low state and the rest the hi kind."""
+    st = _case(2, 8, 8)
+    got, want, r_lo, r_hi = _split_at_edge(st, side, "reflecting", (0.375, 0.0, 0.0, 1.0), 0.0, 3)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, r_lo) and not np.array_equal(got, r_hi)
+
+
+def test_split_side_moves_with_time():
+    """Switch point (1/4 + t) / 2 on the top: at t = 1/4 it sits on x = 1/4 (vertex 2), at
+    t = 3/4 on x = 1/2 (vertex 4)."""
+    st = _case(2, 8, 8)
+    coef = (0.0, 0.25, 1.0, 2.0)
+    for t, k in ((0.25, 2), (0.75, 4)):
+        got, want, _, _ = _split_at_edge(st, "top", "dirichlet", coef, t, k)
+        assert np.array_equal(got, want)
+
+
+def test_split_side_stage_times_of_a_step():
+    """One SSP-RK3 step from t0 = 0 with dt = 1/1024 and the switch point 1/4 + 256 t on the top
+    equals the step composed from residuals at t0, t0 + dt and t0 + dt / 2 (SSP_RK3.jl:40-60):
+    the three stage times put the switch point on three different element edges (x = 1/4,
+    1/2 and 3/8), so a wrong stage time changes the result."""
+    st = _case(1, 8, 8)
+    sides = {"top": _split((0.25, 0.0, 256.0, 1.0), "dirichlet"), "bottom": "reflecting"}
+    orc = _oracle(st, "null", "rusanov", sides)
+    t0, dt = 0.0, 1.0 / 1024.0
+    u0 = st.u.copy(order="F")
+    status, n, t, u = orc.run(u0, t0, t0 + dt, dt=dt)
+    assert (status, n, t) == ("ok", 1, t0 + dt)
+    u1 = u0 + dt * orc.residual(u0, t0)
+    u2 = 0.75 * u0 + 0.25 * u1 + 0.25 * dt * orc.residual(u1, t0 + dt)
+    u3 = (1.0 / 3.0) * u0 + (2.0 / 3.0) * u2 + (2.0 / 3.0) * dt * orc.residual(u2, t0 + dt / 2.0)
+    assert np.array_equal(u, u3)
+    for t_bad in (t0, t0 + dt):
+        wrong = (1.0 / 3.0) * u0 + (2.0 / 3.0) * u2 + (2.0 / 3.0) * dt * orc.residual(u2, t_bad)
+        assert not np.array_equal(u, wrong)

@@ -39,6 +39,11 @@ struct frf_ctx {
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
     frf::MlpPacked* dMlp = nullptr;    // learned sensor weights, when set
     unsigned char* dSolid = nullptr;   // solid-element mask (Mesh2D.solid), when set
+    double* dXn = nullptr;             // face-point coordinates: x of column jx node i, y of row
+    double* dYn = nullptr;             // jy node j (physical_xy), for split sides
+    double split[4][4] = {};           // split sides: switch point a + (b + c t) / d
+    int split_hi[4] = {};              //              kind above it
+    double split_state[4][4] = {};     //              and its Dirichlet state
     double side_state[4][4] = {};      // Dirichlet ghost states (left, right, bottom, top)
     bool side_state_set[4] = {false, false, false, false};
     HostOps ops{};
@@ -90,7 +95,7 @@ extern "C" int32_t frf_create(const frf_problem* pb, frf_ctx** out) {
     if (pb->p < 1 || pb->p > 5) return fail(nullptr, FRF_E_UNSUPPORTED, "degree p=%d not in 1..5", pb->p);
     if (pb->nx < 1 || pb->ny < 1) return fail(nullptr, FRF_E_ARG, "nx, ny must be >= 1");
     if (pb->neq != 4) return fail(nullptr, FRF_E_UNSUPPORTED, "neq=%d: only Euler2D (4) is supported", pb->neq);
-    auto okbc = [](int b) { return b >= FRF_BC_PERIODIC && b <= FRF_BC_DIRICHLET; };
+    auto okbc = [](int b) { return b >= FRF_BC_PERIODIC && b <= FRF_BC_GHOST_SPLIT; };
     if (!okbc(pb->bc_left) || !okbc(pb->bc_right) || !okbc(pb->bc_bottom) || !okbc(pb->bc_top))
         return fail(nullptr, FRF_E_UNSUPPORTED, "unknown boundary kind");
     if ((pb->bc_left == FRF_BC_PERIODIC) != (pb->bc_right == FRF_BC_PERIODIC))
@@ -151,6 +156,8 @@ extern "C" int32_t frf_destroy(frf_ctx* c) {
     if (c->dCtl) (void)hipFree(c->dCtl);
     if (c->dMlp) (void)hipFree(c->dMlp);
     if (c->dSolid) (void)hipFree(c->dSolid);
+    if (c->dXn) (void)hipFree(c->dXn);
+    if (c->dYn) (void)hipFree(c->dYn);
     if (c->hCtl) (void)hipHostFree(c->hCtl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -265,6 +272,33 @@ extern "C" int32_t frf_set_boundary_state(frf_ctx* c, int32_t side, const double
     return FRF_OK;
 }
 
+extern "C" int32_t frf_set_boundary_split(frf_ctx* c, int32_t side, const double* coef,
+                                          const double* U_lo, int32_t kind_hi, const double* U_hi) {
+    if (!c) return FRF_E_ARG;
+    if (side < 0 || side > 3 || !coef || !U_lo)
+        return fail(c, FRF_E_ARG, "frf_set_boundary_split: bad side, coefficients or state");
+    const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
+    if (sides[side] != FRF_BC_GHOST_SPLIT)
+        return fail(c, FRF_E_ARG, "side %d is not a FRF_BC_GHOST_SPLIT side", side);
+    if (kind_hi != FRF_BC_REFLECTING && kind_hi != FRF_BC_DIRICHLET)
+        return fail(c, FRF_E_ARG, "the kind above the switch point must be reflecting or Dirichlet");
+    if (kind_hi == FRF_BC_DIRICHLET && !U_hi) return fail(c, FRF_E_ARG, "no Dirichlet state above the switch point");
+    for (int q = 0; q < 4; ++q) {
+        if (!std::isfinite(coef[q]) || !std::isfinite(U_lo[q]) ||
+            (kind_hi == FRF_BC_DIRICHLET && !std::isfinite(U_hi[q])))
+            return fail(c, FRF_E_ARG, "non-finite split coefficients or states");
+    }
+    if (coef[3] == 0.0) return fail(c, FRF_E_ARG, "split divisor d must be nonzero");
+    for (int q = 0; q < 4; ++q) {
+        c->split[side][q] = coef[q];
+        c->side_state[side][q] = U_lo[q];
+        c->split_state[side][q] = kind_hi == FRF_BC_DIRICHLET ? U_hi[q] : 0.0;
+    }
+    c->split_hi[side] = kind_hi;
+    c->side_state_set[side] = true;
+    return FRF_OK;
+}
+
 extern "C" int32_t frf_set_solid_mask(frf_ctx* c, const uint8_t* mask, int64_t nel) {
     if (!c) return FRF_E_ARG;
     if (!mask) {   // no solid elements
@@ -305,6 +339,7 @@ extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* 
     const int XI = 0, CROSS = n, FW = n + n * n, FE = FW + n, FS = FE + n, FN = FS + n, JAC = FN + n,
               DELTA = JAC + 1;
     std::vector<double> ct((size_t)nx * ts, 0.0), rt((size_t)ny * ts, 0.0);
+    std::vector<double> xn((size_t)nx * n), yn((size_t)ny * n);   // face-point coordinates
     // columns (x direction): X[i] = (xL + xR)/2 + Jx * xi_i   (MeshMetrics2D.jl:49-58)
     for (int k = 0; k < nx; ++k) {
         double* T = &ct[(size_t)k * ts];
@@ -312,6 +347,7 @@ extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* 
         const double dx = xR - xL, Jx = dx / 2.0;
         double X[frf::kMaxNp];
         for (int i = 0; i < n; ++i) X[i] = (xL + xR) / 2.0 + Jx * xi[i];
+        for (int i = 0; i < n; ++i) xn[(size_t)k * n + i] = X[i];
         for (int i = 0; i < n; ++i) {           // x_xi[i] = sum_q D[i,q] X[q]
             double s = 0.0;
             for (int q = 0; q < n; ++q) s += h.D[i * n + q] * X[q];
@@ -351,6 +387,7 @@ extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* 
         const double dy = yT - yB, Jy = dy / 2.0;
         double Y[frf::kMaxNp];
         for (int j = 0; j < n; ++j) Y[j] = (yB + yT) / 2.0 + Jy * xi[j];
+        for (int j = 0; j < n; ++j) yn[(size_t)k * n + j] = Y[j];
         for (int j = 0; j < n; ++j) {           // y_eta[j] = sum_q D[j,q] Y[q]
             double s = 0.0;
             for (int q = 0; q < n; ++q) s += h.D[j * n + q] * Y[q];
@@ -392,6 +429,10 @@ extern "C" int32_t frf_set_geometry(frf_ctx* c, const double* xv, const double* 
     std::vector<float> ctf(ct.begin(), ct.end()), rtf(rt.begin(), rt.end());
     HIPCHK(c, hipMemcpyAsync(c->dCtabF, ctf.data(), ctf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->dRtabF, rtf.data(), rtf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (!c->dXn) HIPCHK(c, hipMalloc(&c->dXn, xn.size() * sizeof(double)));
+    if (!c->dYn) HIPCHK(c, hipMalloc(&c->dYn, yn.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->dXn, xn.data(), xn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dYn, yn.data(), yn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->geom_set = true;
     return FRF_OK;
@@ -427,8 +468,8 @@ static int32_t ready(frf_ctx* c) {
     if (!c->state_set) return fail(c, FRF_E_STATE, "no state uploaded");
     const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
     for (int k = 0; k < 4; ++k)
-        if (sides[k] == FRF_BC_DIRICHLET && !c->side_state_set[k])
-            return fail(c, FRF_E_STATE, "Dirichlet side %d has no state (frf_set_boundary_state)", k);
+        if ((sides[k] == FRF_BC_DIRICHLET || sides[k] == FRF_BC_GHOST_SPLIT) && !c->side_state_set[k])
+            return fail(c, FRF_E_STATE, "side %d has no ghost state (frf_set_boundary_state / _split)", k);
     return FRF_OK;
 }
 
@@ -464,8 +505,16 @@ static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double
     const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
     for (int k = 0; k < 4; ++k) {
         P.side_bc[k] = sides[k];
-        for (int q = 0; q < 4; ++q) P.side_state[k][q] = c->side_state[k][q];
+        P.split_hi[k] = c->split_hi[k];
+        for (int q = 0; q < 4; ++q) {
+            P.side_state[k][q] = c->side_state[k][q];
+            P.split[k][q] = c->split[k][q];
+            P.split_state[k][q] = c->split_state[k][q];
+        }
     }
+    P.xnode = c->dXn;
+    P.ynode = c->dYn;
+    P.t = c->t;
     return P;
 }
 

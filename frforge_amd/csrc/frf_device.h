@@ -165,6 +165,39 @@ __device__ __forceinline__ float persson_sigma_hw(float eh, float et, float s0, 
     return __builtin_amdgcn_rcpf(1.0f + ex);
 }
 
+// Boundary time of a launch: the state time of a residual, and the reference's stage times t0,
+// t0 + dt, t0 + dt/2 inside an SSP-RK3 step (SSP_RK3.jl:40-60).
+__device__ __forceinline__ double boundary_time(const StageParams& P) {
+    if (P.stage == 0) return P.t;
+    const double t0 = P.ctl->t0, dt = P.ctl->dt;
+    return P.stage == 1 ? t0 : (P.stage == 2 ? t0 + dt : t0 + dt / 2.0);
+}
+
+// Kind and Dirichlet state of non-periodic side k at the face point whose coordinate along the
+// side is node[q] (x on the bottom and top, y on the left and right): a split side
+// (a GhostStateBC switching at a + (b + c t) / d, BoundaryConditions.jl:112-130) takes its low
+// state below the switch point and its hi kind from it on.
+__device__ __forceinline__ int side_ghost(const StageParams& P, int k, const double* node, size_t q,
+                                          double UD[4]) {
+#pragma clang fp contract(off)
+    int kind = P.side_bc[k];
+    bool above = false;
+    if (kind == kBcSplit) {
+        const double* c = P.split[k];
+        const double s = node[q], tb = boundary_time(P);
+        if (s < c[0] + (c[1] + c[2] * tb) / c[3]) {
+            kind = kBcDirichlet;
+        } else {
+            kind = P.split_hi[k];
+            above = true;
+        }
+    }
+    // by value: selecting a pointer into the kernel-argument block would move it to scratch
+#pragma unroll
+    for (int c = 0; c < 4; ++c) UD[c] = above ? P.split_state[k][c] : P.side_state[k][c];
+    return kind;
+}
+
 // Step start: loop test, CFL or fixed step, last-step truncation (SSP_RK3.jl:185-189, 126-133).
 __device__ __forceinline__ void dt_step(DevCtl* c) {
     if (c->done != 0) return;

@@ -98,9 +98,16 @@ struct StageParams {
     int resident, ncu;    // blocks of the stage kernel resident per CU, and CUs (start staggering)
     const MlpPacked* mlp; // learned sensor in place of the Persson sigma (p = 3, mixed form), or null
     int side_bc[4];       // kind of the left, right, bottom, top sides (FRF_BC_*), non-periodic axes
-    double side_state[4][4];   // Dirichlet ghost states
+    double side_state[4][4];   // Dirichlet ghost states; split sides: the state below the switch
+    double split[4][4];   // split sides: switch point a + (b + c t) / d along the side
+    int split_hi[4];      // split sides: kind above the switch point (reflecting or Dirichlet)
+    double split_state[4][4];   // ... and its Dirichlet state
+    const double* xnode;  // face-point x of column jx, node i: [jx*Np + i] (split sides), or null
+    const double* ynode;  // face-point y of row jy, node j: [jy*Np + j]
+    double t;             // boundary time of a residual-only launch (stage 0)
     const unsigned char* solid;   // solid-element mask (1 = inactive), non-periodic axes only, or null
 };
+
 
 struct ElemParams {
     const double* u;

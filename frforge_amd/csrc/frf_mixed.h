@@ -605,7 +605,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                 }
                 face_h<FLUX, 0, BR0>(uE, up, gE, gp, tau, g, hE, solO == solN ? 0 : (solN ? 1 : 2));
             } else {
-                boundary_h<FLUX, 0, BR0, false>(P.side_bc[1], P.side_state[1], uE, gE, g, hE);
+                double UD[4];
+                const int kind = side_ghost(P, 1, P.ynode, (size_t)jy * NP + t, UD);
+                boundary_h<FLUX, 0, BR0, false>(kind, UD, uE, gE, g, hE);
             }
         }
         {   // N face of column i = t
@@ -628,7 +630,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                 }
                 face_h<FLUX, 1, BR0>(uN, up, gN, gp, tau, g, hN, solO == solN ? 0 : (solN ? 1 : 2));
             } else {
-                boundary_h<FLUX, 1, BR0, false>(P.side_bc[3], P.side_state[3], uN, gN, g, hN);
+                double UD[4];
+                const int kind = side_ghost(P, 3, P.xnode, (size_t)jx * NP + t, UD);
+                boundary_h<FLUX, 1, BR0, false>(kind, UD, uN, gN, g, hN);
             }
         }
     }
@@ -658,7 +662,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                     const bool sm_ = is_solid(smask, ew + jxW), sp_ = is_solid(smask, ew + tx0);
                     face_h<FLUX, 0, BR0>(um, uo, gm, go, tau, g, fh, sm_ == sp_ ? 0 : (sp_ ? 1 : 2));
                 } else {
-                    boundary_h<FLUX, 0, BR0, true>(P.side_bc[0], P.side_state[0], uo, go, g, fh);
+                    double UD[4];
+                    const int kind = side_ghost(P, 0, P.ynode, (size_t)(ty0 + r) * NP + j, UD);
+                    boundary_h<FLUX, 0, BR0, true>(kind, UD, uo, go, g, fh);
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0x[c * HX + r * NP + j] = fh[c];
@@ -687,7 +693,9 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
                     const bool sp_ = is_solid(smask, ty0 * nx + tx0 + l);
                     face_h<FLUX, 1, BR0>(um, uo, gm, go, tau, g, fh, sm_ == sp_ ? 0 : (sp_ ? 1 : 2));
                 } else {
-                    boundary_h<FLUX, 1, BR0, true>(P.side_bc[2], P.side_state[2], uo, go, g, fh);
+                    double UD[4];
+                    const int kind = side_ghost(P, 2, P.xnode, (size_t)(tx0 + l) * NP + i, UD);
+                    boundary_h<FLUX, 1, BR0, true>(kind, UD, uo, go, g, fh);
                 }
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) sF0y[c * HY + l * NP + i] = fh[c];

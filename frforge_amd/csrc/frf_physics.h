@@ -258,7 +258,7 @@ __device__ __forceinline__ void iflux_axis(const R uL[4], const R uR[4], R gamma
 // exterior_state of a non-periodic side (BoundaryConditions.jl:43-110) for the outward unit
 // normal (nx, ny): kind 2 reflects the velocity (reflect_conserved, with the floored density),
 // kind 3 takes the given state, anything else (transmissive) the interior trace.
-constexpr int kBcReflecting = 2, kBcDirichlet = 3;   // FRF_BC_REFLECTING, FRF_BC_DIRICHLET
+constexpr int kBcReflecting = 2, kBcDirichlet = 3, kBcSplit = 4;   // FRF_BC_*
 
 template <typename R>
 __device__ __forceinline__ void exterior_state(int kind, const R U[4], R nx, R ny, const double* UD,

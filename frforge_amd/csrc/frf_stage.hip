@@ -398,7 +398,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 }
             } else {
                 R ug[4];
-                exterior_state<R>(P.side_bc[1], trE, nxo, nyo, P.side_state[1], ug);
+                double UD[4];
+                const int kind = side_ghost(P, 1, P.ynode, (size_t)jy * NP + j, UD);
+                exterior_state<R>(kind, trE, nxo, nyo, UD, ug);
                 iflux<FLUX>(trE, ug, nxo, nyo, g, fh);
                 if constexpr (BR0) {
 #pragma unroll
@@ -434,7 +436,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     sw = face_s(aw, bw);
                     const R nxw = dv(-aw, sw), nyw = dv(bw, sw);
                     R ug[4];
-                    exterior_state<R>(wallW ? kBcReflecting : P.side_bc[0], trW, nxw, nyw, P.side_state[0], ug);
+                    double UD[4] = {0.0, 0.0, 0.0, 0.0};
+                    const int kind = wallW ? kBcReflecting : side_ghost(P, 0, P.ynode, (size_t)jy * NP + j, UD);
+                    exterior_state<R>(kind, trW, nxw, nyw, UD, ug);
                     iflux<FLUX>(ug, trW, -nxw, -nyw, g, fw);
                 }
 #pragma unroll
@@ -479,7 +483,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 }
             } else {
                 R ug[4];
-                exterior_state<R>(P.side_bc[3], trN, nxo, nyo, P.side_state[3], ug);
+                double UD[4];
+                const int kind = side_ghost(P, 3, P.xnode, (size_t)jx * NP + i, UD);
+                exterior_state<R>(kind, trN, nxo, nyo, UD, ug);
                 iflux<FLUX>(trN, ug, nxo, nyo, g, fh);
                 if constexpr (BR0) {
 #pragma unroll
@@ -515,7 +521,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     ss = face_s(as, bs);
                     const R nxs = dv(as, ss), nys = dv(-bs, ss);
                     R ug[4];
-                    exterior_state<R>(wallS ? kBcReflecting : P.side_bc[2], trS, nxs, nys, P.side_state[2], ug);
+                    double UD[4] = {0.0, 0.0, 0.0, 0.0};
+                    const int kind = wallS ? kBcReflecting : side_ghost(P, 2, P.xnode, (size_t)jx * NP + i, UD);
+                    exterior_state<R>(kind, trS, nxs, nys, UD, ug);
                     iflux<FLUX>(ug, trS, -nxs, -nys, g, fs);
                 }
 #pragma unroll

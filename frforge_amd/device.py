@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libfrforge_hip.so")
 
 FRF_OK = 0
 ERRORS = {-1: "FRF_E_ARG", -2: "FRF_E_STATE", -3: "FRF_E_HIP", -4: "FRF_E_UNSUPPORTED"}
-BC = {"periodic": 0, "transmissive": 1, "reflecting": 2, "dirichlet": 3}
+BC = {"periodic": 0, "transmissive": 1, "reflecting": 2, "dirichlet": 3, "ghost_split": 4}
 SIDES = {"left": 0, "right": 1, "bottom": 2, "top": 3}
 FLUX = {"rusanov": 0, "hllc": 1}
 PRECISION = {"fp64": 0, "fp32_mixed": 1}
@@ -24,7 +24,7 @@ STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
 EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_boundary_state",
-            "frf_set_solid_mask", "frf_set_learned_sensor",
+            "frf_set_boundary_split", "frf_set_solid_mask", "frf_set_learned_sensor",
             "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH):
     L.frf_set_learned_sensor.argtypes = [vp, C.c_void_p, C.c_int64]
     L.frf_set_boundary_state.argtypes = [vp, C.c_int32, dp]
     L.frf_set_solid_mask.argtypes = [vp, C.c_void_p, C.c_int64]
+    L.frf_set_boundary_split.argtypes = [vp, C.c_int32, dp, dp, C.c_int32, dp]
     L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf_residual.argtypes = [vp, dp, C.c_int64]
@@ -209,6 +210,19 @@ class DeviceContext:
         if Ua.shape != (4,):
             raise ValueError("a Dirichlet state has 4 conserved components")
         self._chk(self.lib.frf_set_boundary_state(self.h, SIDES[side], _dp(Ua)), "frf_set_boundary_state")
+
+    def set_boundary_split(self, side: str, coef, U_lo, hi: str, U_hi=None):
+        """Ghost of a "ghost_split" side: U_lo where s < a + (b + c t) / d along the side, else
+        a reflecting wall (hi="reflecting") or U_hi (hi="dirichlet")."""
+        k = np.ascontiguousarray(coef, dtype=np.float64)
+        lo = np.ascontiguousarray(U_lo, dtype=np.float64)
+        if k.shape != (4,) or lo.shape != (4,):
+            raise ValueError("split coefficients (a, b, c, d) and U_lo[4] required")
+        uh = None if U_hi is None else np.ascontiguousarray(U_hi, dtype=np.float64)
+        self._split_keep = (k, lo, uh)
+        self._chk(self.lib.frf_set_boundary_split(self.h, SIDES[side], _dp(k), _dp(lo), BC[hi],
+                                                  None if uh is None else _dp(uh)),
+                  "frf_set_boundary_split")
 
     def set_solid_mask(self, mask):
         """Solid-element mask (Mesh2D.solid: Nel booleans, element e = jx + nx*jy), or None."""

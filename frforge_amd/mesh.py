@@ -57,6 +57,44 @@ class DirichletBC(AbstractBC):
         return f"DirichletBC(state={self.state})" if self.state is not None else "DirichletBC(u_func)"
 
 
+class GhostStateBC(AbstractBC):
+    """User ghost state (BoundaryConditions.jl:31-33,112-130).  The device path runs the form the
+    reduced double-Mach reflection uses (Cases2D_Optional.jl:72-84), built by ``split``: at a face
+    point with coordinate s along the side (x on the bottom and top, y on the left and right) and
+    time t, the ghost is ``U_lo`` where s < a + (b + c t) / d, and otherwise a reflecting wall
+    (``hi="reflecting"``) or the state ``U_hi`` (``hi="dirichlet"``).  A general
+    ``ghost(u_int, nx, ny, x, y, t)`` callable is accepted on the host side only."""
+
+    def __init__(self, ghost=None):
+        self.ghost = ghost
+        self.coef = None
+        self.U_lo = self.U_hi = None
+        self.hi = None
+
+    @classmethod
+    def split(cls, a, b, c, d, U_lo, hi="reflecting", U_hi=None):
+        if hi not in ("reflecting", "dirichlet"):
+            raise ValueError("hi must be 'reflecting' or 'dirichlet'")
+        if hi == "dirichlet" and U_hi is None:
+            raise ValueError("a Dirichlet hi side needs U_hi")
+        if d == 0:
+            raise ValueError("d must be nonzero")
+        bc = cls()
+        bc.coef = (float(a), float(b), float(c), float(d))
+        bc.U_lo = tuple(float(v) for v in U_lo)
+        bc.hi = hi
+        bc.U_hi = None if U_hi is None else tuple(float(v) for v in U_hi)
+        return bc
+
+    def key(self):
+        return (self.coef, self.U_lo, self.hi, self.U_hi)
+
+    def __repr__(self):
+        if self.coef is None:
+            return "GhostStateBC(ghost)"
+        return f"GhostStateBC.split(coef={self.coef}, hi={self.hi!r})"
+
+
 def _linspace(a: float, b: float, n: int) -> np.ndarray:
     """Vertices a + k (b - a) / n, k = 0..n; correctly rounded whenever the reference's
     ``range(a, b; length=n+1)`` is exact (all meshes used here: power-of-two spacings)."""

@@ -22,7 +22,7 @@ import numpy as np
 from .capturing import AbstractCapturingMethod, NullCapturing
 from .device import DeviceBatch, DeviceContext
 from .equations import Euler2D
-from .mesh import DirichletBC, PeriodicBC, ReflectingBC, TransmissiveBC
+from .mesh import DirichletBC, GhostStateBC, PeriodicBC, ReflectingBC, TransmissiveBC
 from .state import SolutionState2D
 
 
@@ -38,6 +38,11 @@ def _bc_name(bc) -> str:
             raise NotImplementedError("the device path takes constant Dirichlet states "
                                       "(DirichletBC(state=U)), not a u_func")
         return "dirichlet"
+    if isinstance(bc, GhostStateBC):
+        if bc.coef is None:
+            raise NotImplementedError("the device path takes split ghost states "
+                                      "(GhostStateBC.split), not a ghost callable")
+        return "ghost_split"
     raise NotImplementedError(f"{type(bc).__name__} is not supported by the device path")
 
 
@@ -60,6 +65,7 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
            tuple(sorted(prm.items())), float(eq.gamma), id(state.ops),
            getattr(method, "weights_path", None),
            tuple((k, b.state) for k, b in _sides(mesh).items() if isinstance(b, DirichletBC)),
+           tuple((k, b.key()) for k, b in _sides(mesh).items() if isinstance(b, GhostStateBC)),
            None if mesh.solid is None else mesh.solid.tobytes())
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
@@ -76,6 +82,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
     for side, bc in _sides(mesh).items():
         if isinstance(bc, DirichletBC):
             dev.set_boundary_state(side, bc.state)
+        elif isinstance(bc, GhostStateBC):
+            dev.set_boundary_split(side, bc.coef, bc.U_lo, bc.hi, bc.U_hi)
     if mesh.solid is not None:
         dev.set_solid_mask(mesh.solid)
     if getattr(method, "weights_path", None) is not None:   # learned sensor (frforge_amd.learned)

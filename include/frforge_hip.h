@@ -33,8 +33,10 @@ extern "C" {
 
 /* Boundary kinds (BoundaryConditions.jl): periodic (both ends of an axis), transmissive (ghost =
  * interior trace), reflecting (slip wall: the ghost mirrors the velocity through the outward
- * normal, reflect_conserved :43-60), Dirichlet (a constant ghost state, frf_set_boundary_state). */
-enum { FRF_BC_PERIODIC = 0, FRF_BC_TRANSMISSIVE = 1, FRF_BC_REFLECTING = 2, FRF_BC_DIRICHLET = 3 };
+ * normal, reflect_conserved :43-60), Dirichlet (a constant ghost state, frf_set_boundary_state),
+ * ghost split (a GhostStateBC whose ghost switches at a moving point, frf_set_boundary_split). */
+enum { FRF_BC_PERIODIC = 0, FRF_BC_TRANSMISSIVE = 1, FRF_BC_REFLECTING = 2, FRF_BC_DIRICHLET = 3,
+       FRF_BC_GHOST_SPLIT = 4 };
 enum { FRF_SIDE_LEFT = 0, FRF_SIDE_RIGHT = 1, FRF_SIDE_BOTTOM = 2, FRF_SIDE_TOP = 3 };
 enum { FRF_FLUX_RUSANOV = 0, FRF_FLUX_HLLC = 1 };
 enum { FRF_FP64 = 0, FRF_FP32_MIXED = 1 };
@@ -85,6 +87,16 @@ int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y
  * DirichletBC with a constant u_func (BoundaryConditions.jl:90-110).  Required before a run for
  * every Dirichlet side. */
 int32_t frf_set_boundary_state(frf_ctx* ctx, int32_t side, const double* U);
+
+/* The ghost of a FRF_BC_GHOST_SPLIT side: GhostStateBC (BoundaryConditions.jl:112-130) in the
+ * form of the reduced double-Mach reflection's bottom and top (Cases2D_Optional.jl:72-84).  At a
+ * face point with coordinate s along the side (x on the bottom and top, y on the left and right;
+ * physical_xy, MeshMetrics2D.jl:49-54) and boundary time t (the stage times t0, t0 + dt,
+ * t0 + dt/2 of SSP_RK3.jl:40-60), the ghost is U_lo[4] where s < coef[0] + (coef[1] + coef[2] t)
+ * / coef[3], and otherwise that of kind_hi: FRF_BC_REFLECTING, or FRF_BC_DIRICHLET with U_hi[4].
+ * Required before a run for every split side. */
+int32_t frf_set_boundary_split(frf_ctx* ctx, int32_t side, const double* coef, const double* U_lo,
+                               int32_t kind_hi, const double* U_hi);
 
 /* Solid-element mask (Mesh2D(...; solid = mask), Mesh2D.jl:31-32,46,65-69): nel bytes in element
  * order e = jx + nx*jy, nonzero = solid.  residual! (Residual2D.jl:140-416) then treats a face

@@ -46,3 +46,37 @@ def test_ffs_case_through_the_oracle():
     solid = st.mesh.solid
     assert np.all(st.u[:, :, solid, :] == st.u[0, 0, solid, :][0])
     assert np.all(np.isfinite(st.u))
+
+
+def test_double_mach_ic_and_states():
+    """The reduced states (Cases2D_Optional.jl:15-26) and the 60-degree front of the initial
+    condition (:33-37): post-shock exactly where x < 1/6 + y / sqrt(3)."""
+    from frforge_amd import Euler2D
+    from frforge_amd.cases2d_optional import double_mach_ic, double_mach_states
+    from frforge_amd.equations import conserved_to_primitives
+    eq = Euler2D(1.4)
+    U_post, U_pre = double_mach_states(eq)
+    rho, u, v, p = conserved_to_primitives(eq, U_post)
+    assert np.isclose(rho, 3.5) and np.isclose(p, 4.5)
+    assert np.isclose(np.hypot(u, v), 1.6) and np.isclose(np.degrees(np.arctan2(v, u)), -30.0)
+    assert np.allclose(conserved_to_primitives(eq, U_pre), (1.4, 0.0, 0.0, 1.0))
+    x = np.array([0.1, 0.3, 0.3, 1.0])
+    y = np.array([0.0, 0.0, 0.3, 0.5])
+    U = double_mach_ic(eq)(x, y)
+    post = x < 1.0 / 6.0 + y / np.sqrt(3.0)
+    assert post.tolist() == [True, False, True, False]
+    assert np.array_equal(U[post], np.broadcast_to(U_post, U[post].shape))
+    assert np.array_equal(U[~post], np.broadcast_to(U_pre, U[~post].shape))
+
+
+def test_optional_suite_through_the_oracle():
+    """run_p33b_optional_suite (Cases2D_Optional.jl:269-330) with the oracle's time loop: both
+    reduced cases finish, as the reference's smoke tests ask (test/test_2d.jl:263-301)."""
+    from frforge_amd.cases2d_optional import run_p33b_optional_suite
+    cases, overall, hard = run_p33b_optional_suite(integrator=oracle_integrator_2d)
+    assert overall and hard == []
+    assert [c["name"] for c in cases] == ["double_mach_reduced_p1_persson_av",
+                                          "forward_facing_step_p1_persson_av"]
+    dmr = cases[0]
+    assert dmr["pass"] and not dmr["diverged"] and not dmr["nan_detected"]
+    assert 1.0 < dmr["metrics"]["ρ_min"] < dmr["metrics"]["ρ_max"] < 10.0

@@ -865,6 +865,14 @@ extern "C" int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch**
             g_create_err = "frf_batch_create: the batched form runs the FP32-mixed form only";
             return FRF_E_UNSUPPORTED;
         }
+        if (c->dSolid) {   // the group kernels are built without the solid-mask face logic
+            g_create_err = "frf_batch_create: instance " + std::to_string(i) + " has a solid mask (single contexts only)";
+            return FRF_E_UNSUPPORTED;
+        }
+        if (ready(c) != FRF_OK) {
+            g_create_err = "frf_batch_create: instance " + std::to_string(i) + ": " + c->err;
+            return FRF_E_STATE;
+        }
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == c) {
                 g_create_err = "frf_batch_create: a context appears twice";

@@ -342,7 +342,9 @@ __device__ __forceinline__ void boundary_h(int kind, const double* UD, const flo
     }
 }
 
-template <int NP, int FLUX, int CAPT, int TX, int TY>
+// SOLID: the launch carries a solid-element mask (P.solid); without one the wall selection of the
+// face phase compiles away.
+template <int NP, int FLUX, int CAPT, int TX, int TY, bool SOLID = false>
 __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const OpsM<NP>& op, int bid) {
     using L = TabLayout<NP>;
     constexpr int CT = L::STRIDE;
@@ -581,7 +583,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
 #pragma unroll
             for (int i = 0; i < NP; ++i) row[c * NPP + i] = -iJy * row[c * NPP + i] - iJx * rowV[i][c];
     }
-    const unsigned char* const smask = P.solid;   // solid-element mask, or null
+    const unsigned char* const smask = SOLID ? P.solid : nullptr;   // solid-element mask, or null
     const bool solO = valid && is_solid(smask, e);
     float hE[kNeq], hN[kNeq];
     if (valid) {

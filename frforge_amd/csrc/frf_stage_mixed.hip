@@ -6,13 +6,13 @@
 
 namespace frf {
 
-template <int NP, int FLUX, int CAPT, int TX, int TY>
+template <int NP, int FLUX, int CAPT, int TX, int TY, bool SOLID>
 __global__ void __launch_bounds__(TX* TY* NP, FRF_MWAVES) stage_mixed_kernel(StageParams P, OpsM<NP> op) {
-    stage_mixed_body<NP, FLUX, CAPT, TX, TY>(P, op, blockIdx.x);
+    stage_mixed_body<NP, FLUX, CAPT, TX, TY, SOLID>(P, op, blockIdx.x);
 }
 
-template <int NP, int FLUX, int CAPT>
-static hipError_t launch_mixed_t(const StageParams& P0, const HostOps& h, hipStream_t s) {
+template <int NP, int FLUX, int CAPT, bool SOLID>
+static hipError_t launch_mixed_v(const StageParams& P0, const HostOps& h, hipStream_t s) {
     constexpr int TX = TileM<NP>::TX, TY = TileM<NP>::TY;
     StageParams P = P0;
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY;
@@ -22,7 +22,7 @@ static hipError_t launch_mixed_t(const StageParams& P0, const HostOps& h, hipStr
         int dev = 0, r = 0;
         hipDeviceProp_t prop{};
         if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, stage_mixed_kernel<NP, FLUX, CAPT, TX, TY>,
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, stage_mixed_kernel<NP, FLUX, CAPT, TX, TY, SOLID>,
                                                          TX * TY * NP, 0) != hipSuccess)
             r = 0;
         ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
@@ -30,9 +30,14 @@ static hipError_t launch_mixed_t(const StageParams& P0, const HostOps& h, hipStr
     }
     P.resident = resident;
     P.ncu = ncu;
-    hipLaunchKernelGGL((stage_mixed_kernel<NP, FLUX, CAPT, TX, TY>), dim3(P.nblk), dim3(TX * TY * NP), 0, s,
-                       P, make_ops_mixed<NP>(h));
+    hipLaunchKernelGGL((stage_mixed_kernel<NP, FLUX, CAPT, TX, TY, SOLID>), dim3(P.nblk), dim3(TX * TY * NP), 0,
+                       s, P, make_ops_mixed<NP>(h));
     return hipGetLastError();
+}
+
+template <int NP, int FLUX, int CAPT>
+static hipError_t launch_mixed_t(const StageParams& P, const HostOps& h, hipStream_t s) {
+    return P.solid ? launch_mixed_v<NP, FLUX, CAPT, true>(P, h, s) : launch_mixed_v<NP, FLUX, CAPT, false>(P, h, s);
 }
 
 template <int NP>

@@ -215,7 +215,7 @@ int32_t frf1d_destroy(frf1d_ctx* ctx);
  * No single reference counterpart: it runs what the reference does as a sweep of independent
  * ssp_rk3! / integrate! calls over meshes, degrees and fluxes (the case runners' loops,
  * Cases2D_Riemann.jl:194-255, and the invent/scoring sweeps) as one job.  Instances are contexts
- * set up as above in the FP32-mixed form.  Instances sharing degree, flux and capturing method
+ * set up as above in the FP32-mixed form, without a solid mask.  Instances sharing degree, flux and capturing method
  * advance with one stage launch per group; the step sequence of all groups is captured once in
  * a HIP graph and replayed.  A batch keeps pointers to its contexts: destroy it first. */
 typedef struct frf_batch frf_batch;

@@ -39,6 +39,7 @@ struct frf_ctx {
     DevCtl* hCtl = nullptr;            // pinned host mirror for polling
     frf::MlpPacked* dMlp = nullptr;    // learned sensor weights, when set
     unsigned char* dSolid = nullptr;   // solid-element mask (Mesh2D.solid), when set
+    double* dMet = nullptr;            // curved metrics (frf_set_curved_metrics), when set
     double* dXn = nullptr;             // face-point coordinates: x of column jx node i, y of row
     double* dYn = nullptr;             // jy node j (physical_xy), for split sides
     double split[4][4] = {};           // split sides: switch point a + (b + c t) / d
@@ -157,6 +158,7 @@ extern "C" int32_t frf_destroy(frf_ctx* c) {
     if (c->dMlp) (void)hipFree(c->dMlp);
     if (c->dSolid) (void)hipFree(c->dSolid);
     if (c->dXn) (void)hipFree(c->dXn);
+    if (c->dMet) (void)hipFree(c->dMet);
     if (c->dYn) (void)hipFree(c->dYn);
     if (c->hCtl) (void)hipHostFree(c->hCtl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -272,6 +274,50 @@ extern "C" int32_t frf_set_boundary_state(frf_ctx* c, int32_t side, const double
     return FRF_OK;
 }
 
+extern "C" int32_t frf_set_curved_metrics(frf_ctx* c, const double* x_xi, const double* x_eta,
+                                          const double* y_xi, const double* y_eta, const double* J,
+                                          const double* nx_W, const double* ny_W, const double* sJ_W,
+                                          const double* nx_E, const double* ny_E, const double* sJ_E,
+                                          const double* nx_S, const double* ny_S, const double* sJ_S,
+                                          const double* nx_N, const double* ny_N, const double* sJ_N) {
+    if (!c) return FRF_E_ARG;
+    const double* vol[5] = {x_xi, x_eta, y_xi, y_eta, J};
+    const double* face[12] = {nx_W, ny_W, sJ_W, nx_E, ny_E, sJ_E, nx_S, ny_S, sJ_S, nx_N, ny_N, sJ_N};
+    if (!x_xi) {   // back to the affine tables
+        if (c->dMet) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(c->dMet));
+            c->dMet = nullptr;
+        }
+        return FRF_OK;
+    }
+    for (auto* a : vol) if (!a) return fail(c, FRF_E_ARG, "frf_set_curved_metrics: null array");
+    for (auto* a : face) if (!a) return fail(c, FRF_E_ARG, "frf_set_curved_metrics: null array");
+    if (c->pb.precision != FRF_FP64)
+        return fail(c, FRF_E_UNSUPPORTED, "curved metrics run in the FP64 form (the FP32-mixed form is direction-separable)");
+    const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
+    for (int k = 0; k < 4; ++k)
+        if (sides[k] == FRF_BC_GHOST_SPLIT)
+            return fail(c, FRF_E_UNSUPPORTED, "split ghost sides take affine face coordinates");
+    const int n = c->np, npp = n * n, nel = c->nel;
+    const frf::CurvOff co = frf::curv_off(n);
+    std::vector<double> m((size_t)nel * co.STRIDE);
+    for (int e = 0; e < nel; ++e) {
+        double* b = &m[(size_t)e * co.STRIDE];
+        for (int k = 0; k < 5; ++k)
+            for (int q = 0; q < npp; ++q) b[k * npp + q] = vol[k][(size_t)e * npp + q];   // (Np,Np,Nel)
+        for (int k = 0; k < 12; ++k)
+            for (int q = 0; q < n; ++q) b[co.NXW + k * n + q] = face[k][(size_t)e * n + q];   // (Np,Nel)
+        for (int q = 0; q < npp; ++q)
+            if (!(b[co.J + q] > 0.0) || !std::isfinite(b[co.J + q]))
+                return fail(c, FRF_E_ARG, "curved metrics: J must be positive and finite (element %d)", e);
+    }
+    if (!c->dMet) HIPCHK(c, hipMalloc(&c->dMet, m.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->dMet, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FRF_OK;
+}
+
 extern "C" int32_t frf_set_boundary_split(frf_ctx* c, int32_t side, const double* coef,
                                           const double* U_lo, int32_t kind_hi, const double* U_hi) {
     if (!c) return FRF_E_ARG;
@@ -289,6 +335,7 @@ extern "C" int32_t frf_set_boundary_split(frf_ctx* c, int32_t side, const double
             return fail(c, FRF_E_ARG, "non-finite split coefficients or states");
     }
     if (coef[3] == 0.0) return fail(c, FRF_E_ARG, "split divisor d must be nonzero");
+    if (c->dMet) return fail(c, FRF_E_UNSUPPORTED, "split ghost sides take affine face coordinates");
     for (int q = 0; q < 4; ++q) {
         c->split[side][q] = coef[q];
         c->side_state[side][q] = U_lo[q];
@@ -512,6 +559,7 @@ static frf::StageParams stage_params(frf_ctx* c, const double* uin, const double
             P.split_state[k][q] = c->split_state[k][q];
         }
     }
+    P.met = c->dMet;
     P.xnode = c->dXn;
     P.ynode = c->dYn;
     P.t = c->t;

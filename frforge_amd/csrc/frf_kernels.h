@@ -1,5 +1,7 @@
 // Device-side structures shared by the kernels and the host library.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace frf {
@@ -106,7 +108,17 @@ struct StageParams {
     const double* ynode;  // face-point y of row jy, node j: [jy*Np + j]
     double t;             // boundary time of a residual-only launch (stage 0)
     const unsigned char* solid;   // solid-element mask (1 = inactive), non-periodic axes only, or null
+    const double* met;    // curved metrics, curv_off(Np).STRIDE values per element (FP64 form), or null
 };
+
+// Per-element block of curved metrics (MeshMetrics2D.jl:15-39): x_xi, x_eta, y_xi, y_eta and J
+// at the Np x Np points (i + Np j), then nx, ny, sJ of the W, E, S and N faces (Np each).
+struct CurvOff {
+    int XXI, XETA, YXI, YETA, J, NXW, STRIDE;
+};
+__host__ __device__ constexpr CurvOff curv_off(int np) {
+    return CurvOff{0, np * np, 2 * np * np, 3 * np * np, 4 * np * np, 5 * np * np, 5 * np * np + 12 * np};
+}
 
 
 struct ElemParams {

@@ -38,7 +38,7 @@ struct Lane {
 
 // Two waves per SIMD are asked of the register allocator for the NullCapturing form, which
 // needs just over 256 registers without the request (one wave per SIMD otherwise).
-template <int NP, typename R, int FLUX, int CAPT, int TX, int TY, bool PF>
+template <int NP, typename R, int FLUX, int CAPT, int TX, int TY, bool PF, bool CURV>
 __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 : 1)
     stage_kernel(StageParams P, OpsT<NP, R> op) {
     using L = TabLayout<NP>;
@@ -221,6 +221,33 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
     const double eps_e = AV ? sEpsT[le] : 0.0;
     const double Jx = ct[L::JAC], dx = ct[L::DELTA], Jy = rt[L::JAC], dy = rt[L::DELTA];
     const R iJx = R(1.0 / Jx), iJy = R(1.0 / Jy);
+    // Metric terms at point (i, j): the affine tables, or this element's curved metrics
+    // (MeshMetrics2D.jl:15-39, given by frf_set_curved_metrics; CURV)
+    constexpr CurvOff CO = curv_off(NP);
+    const double* const me = CURV ? P.met + (size_t)(valid ? e : 0) * CO.STRIDE : nullptr;
+    auto mXxi = [&](int i, int j) -> R {
+        if constexpr (CURV) return R(me[CO.XXI + i + NP * j]); else return ctR[L::XI + i];
+    };
+    auto mYeta = [&](int i, int j) -> R {
+        if constexpr (CURV) return R(me[CO.YETA + i + NP * j]); else return rtR[L::XI + j];
+    };
+    auto mXeta = [&](int i, int j) -> R {
+        if constexpr (CURV) return R(me[CO.XETA + i + NP * j]); else return ctR[L::CROSS + j * NP + i];
+    };
+    auto mYxi = [&](int i, int j) -> R {
+        if constexpr (CURV) return R(me[CO.YXI + i + NP * j]); else return rtR[L::CROSS + j * NP + i];
+    };
+    auto mIJ = [&](int i, int j) -> R {   // 1 / J
+        if constexpr (CURV) return rcp_(R(me[CO.J + i + NP * j]));
+        else return rcp_(ctR[L::XI + i] * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+    };
+    // unit normal and scaling of face point q on side k (0 W, 1 E, 2 S, 3 N) of element en
+    auto curvFace = [&](int en, int k, int q, R& nxf, R& nyf, R& sf) {
+        const double* m = P.met + (size_t)en * CO.STRIDE + CO.NXW + 3 * NP * k;
+        nxf = R(m[q]);
+        nyf = R(m[NP + q]);
+        sf = R(m[2 * NP + q]);
+    };
     R* myU = sU + le * ESTR;
     R volE[NP][kNeq];              // eta: D G~ along column i = t
     R trS[kNeq], trN[kNeq], gS[kNeq], gN[kNeq];
@@ -237,13 +264,12 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) uc[j][c] = myU[c * NPP + j * NP + i];
             R Gt[NP][kNeq];
-            const R xxi = ctR[L::XI + i];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
                 Prim<R> q = primitives(uc[j], g);
                 R F[4], G[4];
                 phys_flux(uc[j], q, F, G);
-                const R yxi = rtR[L::CROSS + j * NP + i];
+                const R xxi = mXxi(i, j), yxi = mYxi(i, j);
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) Gt[j][c] = -yxi * F[c] + xxi * G[c];
             }
@@ -278,13 +304,12 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) ur[i][c] = myU[c * NPP + j * NP + i];
             R Ft[NP][kNeq];
-            const R yeta = rtR[L::XI + j];
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
                 Prim<R> q = primitives(ur[i], g);
                 R F[4], G[4];
                 phys_flux(ur[i], q, F, G);
-                const R xeta = ctR[L::CROSS + j * NP + i];
+                const R yeta = mYeta(i, j), xeta = mXeta(i, j);
 #pragma unroll
                 for (int c = 0; c < kNeq; ++c) Ft[i][c] = yeta * F[c] - xeta * G[c];
             }
@@ -297,7 +322,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 line_deriv<NP>(op.D, Ft, R(1), volX);
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    const R iJ = rcp_(ctR[L::XI + i] * yeta - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+                    const R iJ = mIJ(i, j);
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) rowv[i][c] = -volX[i][c] * iJ;
                 }
@@ -336,10 +361,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
     // -(D G~)/J (+ the eta artificial viscosity); P3 adds the face corrections in place.
     if (valid) {
         const int i = t;
-        const R xxi = ctR[L::XI + i];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-            const R iJ = rcp_(xxi * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+            const R iJ = mIJ(i, j);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
                 R w = volE[j][c] * iJ;
@@ -365,9 +389,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             const bool in_tile = lx + 1 < nvx;
             const bool has_n = in_tile || jxE >= 0;
             const int jxn = in_tile ? jx + 1 : jxE;
-            const R a = rtR[L::FE + j], b = ctR[L::FE + j];   // y_eta, x_eta at the E face
-            R s = face_s(a, b);
-            const R nxo = dv(a, s), nyo = dv(-b, s);
+            R s, nxo, nyo;
+            if constexpr (CURV) {
+                curvFace(e, 1, j, nxo, nyo, s);
+            } else {
+                const R a = rtR[L::FE + j], b = ctR[L::FE + j];   // y_eta, x_eta at the E face
+                s = face_s(a, b);
+                nxo = dv(a, s);
+                nyo = dv(-b, s);
+            }
             R fh[4];
             if (has_n) {
                 const Lane<R> up = in_tile ? Lane<R>{&sFx[(le + 1) * NP + j], FS} : Lane<R>{&sHE[ly * NP + j], TY * NP};
@@ -382,9 +412,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     exterior_state<R>(kBcReflecting, trE, nxo, nyo, nullptr, ug);
                     iflux<FLUX>(trE, ug, nxo, nyo, g, fh);
                 } else {             // wall on the east element's west face, with its metrics
-                    const R aw = rtR[L::FW + j], bw = ctabR[(size_t)jxn * CT + L::FW + j];
-                    s = face_s(aw, bw);
-                    const R nxw = dv(-aw, s), nyw = dv(bw, s);
+                    R nxw, nyw;
+                    if constexpr (CURV) {
+                        curvFace(jy * nx + jxn, 0, j, nxw, nyw, s);
+                    } else {
+                        const R aw = rtR[L::FW + j], bw = ctabR[(size_t)jxn * CT + L::FW + j];
+                        s = face_s(aw, bw);
+                        nxw = dv(-aw, s);
+                        nyw = dv(bw, s);
+                    }
                     R ug[4];
                     exterior_state<R>(kBcReflecting, upv, nxw, nyw, nullptr, ug);
                     iflux<FLUX>(ug, upv, -nxw, -nyw, g, fh);
@@ -415,12 +451,19 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 // a solid west neighbour is a wall on this fluid element's own west face
                 const bool wallW = jxW >= 0 && !solO && is_solid(smask, jy * nx + jxW);
                 if (jxW >= 0) {
-                    const R aw = rtR[L::FE + j], bw = ctabR[(size_t)jxW * CT + L::FE + j];
-                    sw = face_s(aw, bw);
+                    R nxi, nyi;   // the west element's east face
+                    if constexpr (CURV) {
+                        curvFace(jy * nx + jxW, 1, j, nxi, nyi, sw);
+                    } else {
+                        const R aw = rtR[L::FE + j], bw = ctabR[(size_t)jxW * CT + L::FE + j];
+                        sw = face_s(aw, bw);
+                        nxi = dv(aw, sw);
+                        nyi = dv(-bw, sw);
+                    }
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHW[c * (TY * NP) + ly * NP + j];
-                    if (!wallW) iflux<FLUX>(um, trW, dv(aw, sw), dv(-bw, sw), g, fw);
+                    if (!wallW) iflux<FLUX>(um, trW, nxi, nyi, g, fw);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[ly], eps_e, P.ctab[(size_t)jxW * CT + L::DELTA], dx, P.p);
 #pragma unroll
@@ -432,9 +475,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     for (int c = 0; c < kNeq; ++c) sG0x[c * (TY * NP) + ly * NP + j] = gxW[c];
                 }
                 if (jxW < 0 || wallW) {
-                    const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];   // own west face
-                    sw = face_s(aw, bw);
-                    const R nxw = dv(-aw, sw), nyw = dv(bw, sw);
+                    R nxw, nyw;   // own west face
+                    if constexpr (CURV) {
+                        curvFace(e, 0, j, nxw, nyw, sw);
+                    } else {
+                        const R aw = rtR[L::FW + j], bw = ctR[L::FW + j];
+                        sw = face_s(aw, bw);
+                        nxw = dv(-aw, sw);
+                        nyw = dv(bw, sw);
+                    }
                     R ug[4];
                     double UD[4] = {0.0, 0.0, 0.0, 0.0};
                     const int kind = wallW ? kBcReflecting : side_ghost(P, 0, P.ynode, (size_t)jy * NP + j, UD);
@@ -450,9 +499,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             const bool in_tile = ly + 1 < nvy;
             const bool has_n = in_tile || jyN >= 0;
             const int jyn = in_tile ? jy + 1 : jyN;
-            const R a = rtR[L::FN + i], b = ctR[L::FN + i];   // y_xi, x_xi at the N face
-            R s = face_s(a, b);
-            const R nxo = dv(-a, s), nyo = dv(b, s);
+            R s, nxo, nyo;
+            if constexpr (CURV) {
+                curvFace(e, 3, i, nxo, nyo, s);
+            } else {
+                const R a = rtR[L::FN + i], b = ctR[L::FN + i];   // y_xi, x_xi at the N face
+                s = face_s(a, b);
+                nxo = dv(-a, s);
+                nyo = dv(b, s);
+            }
             R fh[4];
             if (has_n) {
                 const Lane<R> up = in_tile ? Lane<R>{&sFy[(le + TX) * NP + i], FS} : Lane<R>{&sHN[lx * NP + i], TX * NP};
@@ -467,9 +522,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     exterior_state<R>(kBcReflecting, trN, nxo, nyo, nullptr, ug);
                     iflux<FLUX>(trN, ug, nxo, nyo, g, fh);
                 } else {             // wall on the north element's south face, with its metrics
-                    const R as = rtabR[(size_t)jyn * CT + L::FS + i], bs = ctR[L::FS + i];
-                    s = face_s(as, bs);
-                    const R nxs = dv(as, s), nys = dv(-bs, s);
+                    R nxs, nys;
+                    if constexpr (CURV) {
+                        curvFace(jyn * nx + jx, 2, i, nxs, nys, s);
+                    } else {
+                        const R as = rtabR[(size_t)jyn * CT + L::FS + i], bs = ctR[L::FS + i];
+                        s = face_s(as, bs);
+                        nxs = dv(as, s);
+                        nys = dv(-bs, s);
+                    }
                     R ug[4];
                     exterior_state<R>(kBcReflecting, upv, nxs, nys, nullptr, ug);
                     iflux<FLUX>(ug, upv, -nxs, -nys, g, fh);
@@ -500,12 +561,19 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 // a solid south neighbour is a wall on this fluid element's own south face
                 const bool wallS = jyS >= 0 && !solO && is_solid(smask, jyS * nx + jx);
                 if (jyS >= 0) {
-                    const R as = rtabR[(size_t)jyS * CT + L::FN + i], bs = ctR[L::FN + i];
-                    ss = face_s(as, bs);
+                    R nxi, nyi;   // the south element's north face
+                    if constexpr (CURV) {
+                        curvFace(jyS * nx + jx, 3, i, nxi, nyi, ss);
+                    } else {
+                        const R as = rtabR[(size_t)jyS * CT + L::FN + i], bs = ctR[L::FN + i];
+                        ss = face_s(as, bs);
+                        nxi = dv(-as, ss);
+                        nyi = dv(bs, ss);
+                    }
                     R um[kNeq];
 #pragma unroll
                     for (int c = 0; c < kNeq; ++c) um[c] = sHS[c * (TX * NP) + lx * NP + i];
-                    if (!wallS) iflux<FLUX>(um, trS, dv(-as, ss), dv(bs, ss), g, fs);
+                    if (!wallS) iflux<FLUX>(um, trS, nxi, nyi, g, fs);
                     if constexpr (BR0) {
                         const R tau = br0_tau<R>(sEpsH[2 * TY + lx], eps_e, P.rtab[(size_t)jyS * CT + L::DELTA], dy, P.p);
 #pragma unroll
@@ -517,9 +585,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                     for (int c = 0; c < kNeq; ++c) sG0y[c * (TX * NP) + lx * NP + i] = gyS[c];
                 }
                 if (jyS < 0 || wallS) {
-                    const R as = rtR[L::FS + i], bs = ctR[L::FS + i];   // own south face
-                    ss = face_s(as, bs);
-                    const R nxs = dv(as, ss), nys = dv(-bs, ss);
+                    R nxs, nys;   // own south face
+                    if constexpr (CURV) {
+                        curvFace(e, 2, i, nxs, nys, ss);
+                    } else {
+                        const R as = rtR[L::FS + i], bs = ctR[L::FS + i];
+                        ss = face_s(as, bs);
+                        nxs = dv(as, ss);
+                        nys = dv(-bs, ss);
+                    }
                     R ug[4];
                     double UD[4] = {0.0, 0.0, 0.0, 0.0};
                     const int kind = wallS ? kBcReflecting : side_ghost(P, 2, P.xnode, (size_t)jx * NP + i, UD);
@@ -566,10 +640,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 hN[c] = gn[c] - gyN[c];
             }
         }
-        const R xxi = ctR[L::XI + i];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-            const R iJ = rcp_(xxi * rtR[L::XI + j] - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+            const R iJ = mIJ(i, j);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
                 R v = myB[c * NPP + j * NP + i] + (fS[c] * op.gL[j] + fN[c] * op.gR[j]) * iJ;
@@ -602,11 +675,10 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
                 hE[c] = ge[c] - gxE[c];
             }
         }
-        const R yeta = rtR[L::XI + j];
         R du[NP][kNeq];
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            const R iJ = rcp_(ctR[L::XI + i] * yeta - ctR[L::CROSS + j * NP + i] * rtR[L::CROSS + j * NP + i]);
+            const R iJ = mIJ(i, j);
 #pragma unroll
             for (int c = 0; c < kNeq; ++c) {
                 R d = rowv[i][c] - (fW[c] * op.gL[i] + fE[c] * op.gR[i]) * iJ - myB[c * NPP + j * NP + i];
@@ -764,8 +836,12 @@ static hipError_t launch_stage_t(const StageParams& P0, const HostOps& h, hipStr
     P.nblk = ntx * nty;
     // u^n is prefetched with u^(k) where the stage has enough arithmetic to hide it (capturing)
     constexpr bool PF = FRF_PREFETCH_U0 < 0 ? (CAPT != 0) : (FRF_PREFETCH_U0 != 0);
-    hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY, PF>), dim3(P.nblk), dim3(TX * TY * NP), 0, s, P,
-                       make_ops<NP, R>(h));
+    if (P.met)   // curved metrics
+        hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY, PF, true>), dim3(P.nblk), dim3(TX * TY * NP), 0,
+                           s, P, make_ops<NP, R>(h));
+    else
+        hipLaunchKernelGGL((stage_kernel<NP, R, FLUX, CAPT, TX, TY, PF, false>), dim3(P.nblk), dim3(TX * TY * NP),
+                           0, s, P, make_ops<NP, R>(h));
     return hipGetLastError();
 }
 

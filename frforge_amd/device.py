@@ -24,7 +24,8 @@ STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
 
 # Every extern "C" symbol the header declares (checked by the CPU test suite).
 EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_boundary_state",
-            "frf_set_boundary_split", "frf_set_solid_mask", "frf_set_learned_sensor",
+            "frf_set_boundary_split", "frf_set_solid_mask", "frf_set_curved_metrics",
+            "frf_set_learned_sensor",
             "frf_upload_state",
             "frf_download_state", "frf_residual", "frf_sense", "frf_compute_dt", "frf_rk3_step",
             "frf_run", "frf_advance", "frf_profile_steps", "frf_device_count",
@@ -89,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     L.frf_set_boundary_state.argtypes = [vp, C.c_int32, dp]
     L.frf_set_solid_mask.argtypes = [vp, C.c_void_p, C.c_int64]
     L.frf_set_boundary_split.argtypes = [vp, C.c_int32, dp, dp, C.c_int32, dp]
+    L.frf_set_curved_metrics.argtypes = [vp] + [dp] * 17
     L.frf_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf_residual.argtypes = [vp, dp, C.c_int64]
@@ -223,6 +225,17 @@ class DeviceContext:
         self._chk(self.lib.frf_set_boundary_split(self.h, SIDES[side], _dp(k), _dp(lo), BC[hi],
                                                   None if uh is None else _dp(uh)),
                   "frf_set_boundary_split")
+
+    _CURVED = ("x_xi", "x_eta", "y_xi", "y_eta", "J", "nx_W", "ny_W", "sJ_W", "nx_E", "ny_E",
+               "sJ_E", "nx_S", "ny_S", "sJ_S", "nx_N", "ny_N", "sJ_N")
+
+    def set_curved_metrics(self, metrics):
+        """Curved metrics (a MeshMetrics2D with face arrays), or None for the affine tables."""
+        if metrics is None:
+            self._chk(self.lib.frf_set_curved_metrics(self.h, *([None] * 17)), "frf_set_curved_metrics")
+            return
+        arrs = [np.asfortranarray(getattr(metrics, n), dtype=np.float64) for n in self._CURVED]
+        self._chk(self.lib.frf_set_curved_metrics(self.h, *[_dp(a) for a in arrs]), "frf_set_curved_metrics")
 
     def set_solid_mask(self, mask):
         """Solid-element mask (Mesh2D.solid: Nel booleans, element e = jx + nx*jy), or None."""

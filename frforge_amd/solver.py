@@ -66,7 +66,8 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
            getattr(method, "weights_path", None),
            tuple((k, b.state) for k, b in _sides(mesh).items() if isinstance(b, DirichletBC)),
            tuple((k, b.key()) for k, b in _sides(mesh).items() if isinstance(b, GhostStateBC)),
-           None if mesh.solid is None else mesh.solid.tobytes())
+           None if mesh.solid is None else mesh.solid.tobytes(),
+           id(state.metrics) if state.metrics.is_curved else None)
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
         return dev
@@ -86,6 +87,14 @@ def _device(state: SolutionState2D, eq: Euler2D, method: AbstractCapturingMethod
             dev.set_boundary_split(side, bc.coef, bc.U_lo, bc.hi, bc.U_hi)
     if mesh.solid is not None:
         dev.set_solid_mask(mesh.solid)
+    if state.metrics.is_curved:
+        jx = np.arange(mesh.n_elements) % mesh.nx
+        jy = np.arange(mesh.n_elements) // mesh.nx
+        if not np.array_equal(state.metrics.h_char, np.minimum(mesh.dx[jx], mesh.dy[jy])):
+            raise NotImplementedError("the device keeps the affine element size min(dx, dy); curved "
+                                      "metrics with another h_char (isoparametric D-metrics) are not "
+                                      "supported")
+        dev.set_curved_metrics(state.metrics)
     if getattr(method, "weights_path", None) is not None:   # learned sensor (frforge_amd.learned)
         dev.set_learned_sensor(method.weights().pack())
     dev.key = key

@@ -12,7 +12,7 @@ from typing import Any, Callable, Optional
 import numpy as np
 
 from .equations import Euler2D, pressure
-from .mesh import Mesh2D, MeshMetrics2D, build_mesh_metrics
+from .mesh import Mesh2D, MeshMetrics2D, build_mesh_metrics, build_mesh_metrics_analytic_wavy
 from .operators import FROperators
 
 
@@ -57,14 +57,22 @@ class SolutionState2D:
 
 
 def allocate_state(mesh: Mesh2D, ops: FROperators, neq: int = 4,
-                   scheme: Optional[SchemeConfig] = None, precision: str = "fp64") -> SolutionState2D:
-    """(SolutionState2D.jl:20-43).  Only Neq = 4 (2D Euler) runs on the device."""
+                   scheme: Optional[SchemeConfig] = None, precision: str = "fp64",
+                   metrics=None, wavy_amp: Optional[float] = None) -> SolutionState2D:
+    """(SolutionState2D.jl:20-43): the given metrics, else the analytic wavy ones when
+    ``wavy_amp`` is set, else build_mesh_metrics (affine, or the discrete metrics of a wavy mesh).
+    Only Neq = 4 (2D Euler) runs on the device; curved metrics run in the FP64 form."""
     if precision not in ("fp64", "fp32_mixed"):
         raise ValueError(f"precision must be 'fp64' or 'fp32_mixed', got {precision}")
     Np = ops.n_points
     u = np.zeros((Np, Np, mesh.n_elements, neq), order="F")
     sch = scheme if scheme is not None else SchemeConfig(points=ops.points)
-    met = build_mesh_metrics(mesh, ops)
+    if metrics is not None:
+        met = metrics
+    elif wavy_amp is not None:
+        met = build_mesh_metrics_analytic_wavy(mesh, ops, float(wavy_amp))
+    else:
+        met = build_mesh_metrics(mesh, ops)
     return SolutionState2D(u, 0.0, ops.p, mesh, ops, sch, met, precision)
 
 

@@ -88,6 +88,19 @@ int32_t frf_set_geometry(frf_ctx* ctx, const double* x_vertices, const double* y
  * every Dirichlet side. */
 int32_t frf_set_boundary_state(frf_ctx* ctx, int32_t side, const double* U);
 
+/* Curved metrics (MeshMetrics2D.jl:15-39), e.g. build_mesh_metrics_analytic_wavy's
+ * (:320-442): x_xi, x_eta, y_xi, y_eta, J as (Np,Np,Nel) and the unit normals and scalings of the
+ * four faces as (Np,Nel), in the reference's layout.  The residual then takes its volume terms
+ * and face normals from them instead of the affine tables; BR0, local DD, the element size and dt
+ * keep the affine Jx, Jy and min(dx, dy), as the reference's do for analytic wavy metrics.  FP64
+ * form only, not with split sides; x_xi = NULL returns to the affine metrics. */
+int32_t frf_set_curved_metrics(frf_ctx* ctx, const double* x_xi, const double* x_eta,
+                               const double* y_xi, const double* y_eta, const double* J,
+                               const double* nx_W, const double* ny_W, const double* sJ_W,
+                               const double* nx_E, const double* ny_E, const double* sJ_E,
+                               const double* nx_S, const double* ny_S, const double* sJ_S,
+                               const double* nx_N, const double* ny_N, const double* sJ_N);
+
 /* The ghost of a FRF_BC_GHOST_SPLIT side: GhostStateBC (BoundaryConditions.jl:112-130) in the
  * form of the reduced double-Mach reflection's bottom and top (Cases2D_Optional.jl:72-84).  At a
  * face point with coordinate s along the side (x on the bottom and top, y on the left and right;

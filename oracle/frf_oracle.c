@@ -175,6 +175,7 @@ struct orc_ctx {
     int split_hi[4];                  /* split sides: kind above the switch point */
     double split_state[4][4];         /* ... and its Dirichlet state */
     double t_res;                     /* time of the residual being evaluated */
+    int curved;                       /* metrics given by orc_set_metrics */
     int Np, Nel;
     double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
     double *LU; int* piv;             /* LU of V with partial pivoting (sensor solves) */
@@ -279,6 +280,23 @@ static void build_metrics(orc_ctx* h) {
         int jx = e % nx, jy = e / nx;
         h->met[19][e] = jmin(h->dx[jx], h->dy[jy]);   /* affine h_char (MeshMetrics2D.jl:206-209) */
     }
+}
+
+/* Metrics given by the caller in MeshMetrics2D's field order (MeshMetrics2D.jl:15-39): x, y,
+ * x_xi, x_eta, y_xi, y_eta, J as (Np,Np,Nel), nx, ny, sJ of the W, E, S, N faces as (Np,Nel),
+ * and h_char (Nel).  The residual, sensor and time step read only these arrays, so curved
+ * metrics need no other change. */
+int orc_set_metrics(orc_ctx* h, const double* const* arrays) {
+    for (int k = 0; k < 4; ++k)
+        if (h->bc_kind[k] == ORC_BC_SPLIT) return -1;   /* split sides take affine coordinates */
+    size_t nv = (size_t)h->Np * h->Np * h->Nel, nf = (size_t)h->Np * h->Nel;
+    for (int k = 0; k < 20; ++k)
+        if (!arrays[k]) return -1;
+    for (int k = 0; k < 7; ++k) memcpy(h->met[k], arrays[k], nv * sizeof(double));
+    for (int k = 7; k < 19; ++k) memcpy(h->met[k], arrays[k], nf * sizeof(double));
+    memcpy(h->met[19], arrays[19], (size_t)h->Nel * sizeof(double));
+    h->curved = 1;
+    return 0;
 }
 
 /* LU with partial pivoting of V (column-major), used for the sensor's V \ U solves. */
@@ -494,7 +512,7 @@ int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U) {
  * (reflecting, or Dirichlet with U_hi). */
 int orc_set_boundary_split(orc_ctx* h, int side, const double* coef, const double* U_lo,
                            int kind_hi, const double* U_hi) {
-    if (side < 0 || side > 3 || !coef || !U_lo) return -1;
+    if (side < 0 || side > 3 || !coef || !U_lo || h->curved) return -1;
     int axis_bc = side < 2 ? h->pb.bc_x : h->pb.bc_y;
     if (axis_bc == ORC_BC_PERIODIC) return -1;
     if (kind_hi != ORC_BC_REFLECTING && kind_hi != ORC_BC_DIRICHLET) return -1;

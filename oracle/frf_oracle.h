@@ -69,6 +69,11 @@ int orc_set_boundary(orc_ctx* h, int side, int kind, const double* U);
  * BoundaryConditions.jl:112-130, as Cases2D_Optional.jl:72-84 uses it): at a face point with
  * coordinate s along the side and time t, U_lo where s < coef[0] + (coef[1] + coef[2] t) /
  * coef[3], otherwise kind_hi (ORC_BC_REFLECTING, or ORC_BC_DIRICHLET with U_hi). */
+/* Metrics in MeshMetrics2D's field order (20 arrays: x, y, x_xi, x_eta, y_xi, y_eta, J, nx, ny,
+ * sJ of the W, E, S, N faces, h_char), e.g. build_mesh_metrics_analytic_wavy's
+ * (MeshMetrics2D.jl:320-442); not with split sides. */
+int orc_set_metrics(orc_ctx* h, const double* const* arrays);
+
 int orc_set_boundary_split(orc_ctx* h, int side, const double* coef, const double* U_lo,
                            int kind_hi, const double* U_hi);
 /* Solid-element mask (Mesh2D.solid, Mesh2D.jl:31-95): faces between a fluid and a solid element

@@ -64,6 +64,8 @@ def lib():
         L.orc_set_solid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.orc_set_boundary.restype = C.c_int
         L.orc_set_boundary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_set_metrics.restype = C.c_int
+        L.orc_set_metrics.argtypes = [C.c_void_p, C.POINTER(dp)]
         L.orc_set_boundary_split.restype = C.c_int
         L.orc_set_boundary_split.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_int, dp]
         L.orc_sense.argtypes = [C.c_void_p, dp, dp]
@@ -136,7 +138,7 @@ class Oracle:
 
     def __init__(self, p, nx, ny, xv, yv, ops, bc_x="periodic", bc_y="periodic",
                  flux="rusanov", capturing="null", gamma=1.4, kappa=4.0, s0_factor=-4.0,
-                 eps_floor=1e-16, c_av=0.1, sides=None, solid=None):
+                 eps_floor=1e-16, c_av=0.1, sides=None, solid=None, metrics=None):
         """sides: optional {"left"|"right"|"bottom"|"top": "transmissive" | "reflecting" |
         ("dirichlet", U[4]) | ("split", (a, b, c, d), U_lo, "reflecting" | "dirichlet", U_hi)}
         for the non-periodic axes (split: U_lo where s < a + (b + c t) / d along the side)."""
@@ -172,6 +174,14 @@ class Oracle:
             if lib().orc_set_boundary(self.h, self.SIDE[side], self.BC_KIND[kind],
                                       None if Ua is None else _p(Ua)) != 0:
                 raise ValueError(f"orc_set_boundary({side}, {kind}) failed")
+        self._metrics = None
+        if metrics is not None:   # MeshMetrics2D-like: the 20 arrays in the reference's field order
+            names = ("x", "y", "x_xi", "x_eta", "y_xi", "y_eta", "J", "nx_W", "ny_W", "sJ_W",
+                     "nx_E", "ny_E", "sJ_E", "nx_S", "ny_S", "sJ_S", "nx_N", "ny_N", "sJ_N", "h_char")
+            self._metrics = [np.asfortranarray(getattr(metrics, n), dtype=np.float64) for n in names]
+            arr = (C.POINTER(C.c_double) * 20)(*[_p(a) for a in self._metrics])
+            if lib().orc_set_metrics(self.h, arr) != 0:
+                raise ValueError("orc_set_metrics failed (split sides take affine coordinates)")
         self._solid = None
         if solid is not None:
             self._solid = np.ascontiguousarray(np.asarray(solid, dtype=bool).astype(np.uint8))

@@ -84,7 +84,13 @@ def oracle_for_state(state, method, eq=EQ):
              if not isinstance(b, PeriodicBC)}
     return Oracle(state.p, m.nx, m.ny, m.x_vertices, m.y_vertices, oracle_ops(state.ops), bc_x=bx,
                   bc_y=by, flux=state.scheme.flux, capturing=kind, gamma=eq.gamma, sides=sides,
-                  solid=m.solid, **prm)
+                  solid=m.solid, metrics=state.metrics if state.metrics.is_curved else None, **prm)
+
+
+def oracle_residual(du, state, eq, method):
+    """residual! through the oracle (the case runners' residual_fn hook)."""
+    du[...] = oracle_for_state(state, method, eq).residual(state.u, state.t)
+    return du
 
 
 def oracle_integrator_2d(state, eq, method, t_final, cfl, dt=None):

@@ -109,3 +109,25 @@ def test_riemann_512_p3_mixed_vs_fp64():
         s_.device.close()
         s_.device = None
     assert rows and rows[0][0] == 0.01 and rows[0][1] <= 1e-6
+
+
+def test_euler2d_smooth_order_on_the_device():
+    """test_2d.jl:30-36: the p = 2 density-wave order study on 16^2 and 32^2 reaches its formal
+    order; the device's L2 errors equal the oracle's to 1e-10 relative."""
+    from frforge_amd.cases2d_smooth import run_euler2d_smooth_order
+    from helpers import oracle_integrator_2d
+    kw = dict(p=2, n_list=(16, 32), t_final=0.25, cfl=0.06)
+    c = run_euler2d_smooth_order(**kw)
+    o = run_euler2d_smooth_order(**kw, integrator=oracle_integrator_2d)
+    print(f"euler2d order: {c['observed_orders']} L2 {c['l2_errors']} (oracle {o['l2_errors']})", flush=True)
+    assert not c["diverged"] and c["order_pass"] and c["positivity_ok"]
+    assert np.allclose(c["l2_errors"], o["l2_errors"], rtol=1e-10, atol=0)
+    assert c["conservation_pass"] == o["conservation_pass"]
+
+
+def test_m8_and_p31_suites_on_the_device():
+    from frforge_amd.cases2d_smooth import run_m8_2d_suite, run_p31_2d_capturing_suite
+    cases, overall, hard, _, _ = run_m8_2d_suite()
+    assert overall and hard == [], hard
+    cases, overall, hard = run_p31_2d_capturing_suite()
+    assert overall and hard == [], hard

@@ -62,6 +62,22 @@
 
 namespace frf {
 
+// Hand-over of LDS data that only an element's own Np line threads read.  When Np divides the
+// wave size an element's lanes are consecutive lanes of one wave, whose LDS instructions execute
+// in order: the wavefront-scope fences (which keep the compiler from moving accesses across) are
+// enough, and waves do not wait for each other.  Otherwise elements straddle waves and the
+// workgroup barrier stays.
+template <int NP>
+__device__ __forceinline__ void elem_sync() {
+    if constexpr (64 % NP == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // ---- Line operators that are exact on constant lines.
 // With d_k = v_k - v_0 a Lagrange trace (weights summing to one) is v_0 + sum_{k>=1} w_k d_k
 // and a derivative row (weights summing to zero) is sum_{k>=1} w_k d_k.  The two forms agree
@@ -543,7 +559,10 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
         }
     }
     if constexpr (FRF_PRIO > 0 && CAPT == 0) __builtin_amdgcn_s_setprio(0);
-    __syncthreads();
+    // P1 reads only this element's slot, written by its own lanes; what P2 reads across waves
+    // (halo lines, traces, viscosities) is ordered by the barrier after P1.  Waves without halo
+    // tasks may start P1 while the others finish theirs.
+    elem_sync<NP>();
 
     // ------------------------------------------------------------------ P1: eta sweep
     float HS[kNeq], HN[kNeq], uN[kNeq], gN[kNeq];
@@ -731,7 +750,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             for (int c = 0; c < kNeq; ++c)
                 col[c * NPP + j * NP] += dS[c] * op.gL[j] + dN[c] * op.gR[j];
     }
-    __syncthreads();
+    elem_sync<NP>();   // the row pass reads this element's slot (neighbours' fluxes: barrier above)
 
     bool bad = false, bad_u = false;
     float lam_row = 0.0f;
@@ -776,15 +795,15 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             }
         }
     }
-    if (P.stage > 0) {
-        const int any_bad = __syncthreads_or(bad ? 1 : 0);
-        const int any_bad_u = (P.stage == 3) ? __syncthreads_or(bad_u ? 1 : 0) : 0;
-        if (tid == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
+    if (P.stage > 0) {   // per wave: the minimum over waves is the block's (stage < 4)
+        const bool any_bad = __ballot(bad) != 0;
+        const bool any_bad_u = P.stage == 3 && __ballot(bad_u) != 0;
+        if ((tid & 63) == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
     }
 
     // ------------------------------------------------------------------ P4: epilogue
     if (P.stage == 0 || !(P.want_lambda || AV)) return;
-    __syncthreads();   // the row reads of sU are done: its element slots become scratch
+    elem_sync<NP>();   // the row reads of this element's slot are done: it becomes scratch
     float* sT = sU + le * ESTR;   // [a*NP + j] row-transformed density, [NPP + k] lambda,
                                   // [NPP + NP + k] total and [NPP + 2 NP + k] high modal energy
     if (valid) {
@@ -799,7 +818,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
         }
         sT[NPP + t] = lam_row;
     }
-    __syncthreads();
+    elem_sync<NP>();
     if constexpr (AV) {
         if (valid) {   // modes (a, b = t): energies of column b of the modal coefficients
             const int b = t;
@@ -820,9 +839,11 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             sT[NPP + NP + t] = et;
             sT[NPP + 2 * NP + t] = eh;
         }
-        __syncthreads();
+        elem_sync<NP>();
         if constexpr (LEARN) {
             if (P.mlp) {   // learned sensor: wave w evaluates elements 16 w .. 16 w + 15
+                // its scratch is the face-flux array the other waves' row passes read
+                __syncthreads();
                 const int w = tid >> 6, lane = tid & 63;
                 const mlp_f4 o = mlp_wave16(P.mlp, sU + (16 * w + (lane & 15)) * ESTR + kModes,
                                             float(P.eps_floor), reinterpret_cast<short*>(sXW) + w * 512);
@@ -830,7 +851,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
 #pragma unroll
                     for (int r = 0; r < 4; ++r) sU[(16 * w + 4 * (lane >> 4) + r) * ESTR + kModes + NPP] = o[r];
                 }
-                __syncthreads();
+                elem_sync<NP>();
             }
         }
     }

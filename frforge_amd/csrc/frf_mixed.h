@@ -381,12 +381,20 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
     constexpr int kModes = NPP + 3 * NP;
     static_assert(!LEARN || kModes + NPP + 1 <= ESTR, "modes fit an element slot");
     static_assert(!LEARN || kNeq * FS * 4 >= NW * 1024, "MLP scratch fits the face arrays");
+    static_assert(!LEARN || NP == 2 || NP == 4, "the MLP scratch needs the separate flux slots");
     // contractions on the matrix cores for Np <= 4 (one 4-row group per matrix)
     constexpr bool MF = FRF_MFMA && NP <= 4 && NTHR % 64 == 0;
 
     __shared__ float sU[NT * ESTR];             // rows of u^(k); eta terms per column; scratch
-    __shared__ float sXW[kNeq * FS];            // W-face u traces, then E-face fluxes H^
-    __shared__ float sYS[kNeq * FS];            // S-face u traces, then N-face fluxes H^
+    // Face fluxes get slots of their own where the LDS budget keeps the resident blocks per CU
+    // (Np = 2, 4); otherwise they overwrite the trace slots after a barrier.
+    constexpr bool SEPF = NP == 2 || NP == 4;
+    __shared__ float sXW[kNeq * FS];            // W-face u traces (then the MLP's scratch)
+    __shared__ float sYS[kNeq * FS];            // S-face u traces
+    __shared__ float sXFs[SEPF ? kNeq * FS : 1];   // E-face fluxes H^
+    __shared__ float sYFs[SEPF ? kNeq * FS : 1];   // N-face fluxes H^
+    float* const sXF = SEPF ? sXFs : sXW;
+    float* const sYF = SEPF ? sYFs : sYS;
     __shared__ float sXG[BR0 ? kNeq * FS : 1];  // BR0: W-face g traces
     __shared__ float sYG[BR0 ? kNeq * FS : 1];  //      S-face g traces
     __shared__ float sHW[kNeq * HX], sHE[kNeq * HX];   // halo u traces: W halo east, E halo west
@@ -723,19 +731,20 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
             }
         }
     }
-    __syncthreads();   // every trace read is done: the trace slots now take the face fluxes
+    // in slots of their own the fluxes need not wait for the neighbours' trace reads
+    if constexpr (!SEPF) __syncthreads();
     if (valid) {
 #pragma unroll
         for (int c = 0; c < kNeq; ++c) {
-            sXW[c * FS + le * NP + t] = hE[c];
-            sYS[c * FS + le * NP + t] = hN[c];
+            sXF[c * FS + le * NP + t] = hE[c];
+            sYF[c * FS + le * NP + t] = hN[c];
         }
     }
-    __syncthreads();
+    __syncthreads();   // every face flux is written and every trace read
 
     // ------------------------------------------------------------------ P3: corrections, combine
     if (valid) {   // column i = t: eta corrections added to the eta volume term in place
-        const float* hs = ly > 0 ? &sYS[(le - TX) * NP + t] : &sF0y[lx * NP + t];
+        const float* hs = ly > 0 ? &sYF[(le - TX) * NP + t] : &sF0y[lx * NP + t];
         const int cs = ly > 0 ? FS : HY;
         float dS[kNeq], dN[kNeq];
 #pragma unroll
@@ -756,7 +765,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
     float lam_row = 0.0f;
     float rho_new[NP];
     if (valid) {   // row j = t
-        const float* hw = lx > 0 ? &sXW[(le - 1) * NP + t] : &sF0x[ly * NP + t];
+        const float* hw = lx > 0 ? &sXF[(le - 1) * NP + t] : &sF0x[ly * NP + t];
         const int cs = lx > 0 ? FS : HX;
         float dW[kNeq], dE[kNeq];
 #pragma unroll
@@ -842,8 +851,7 @@ __device__ __forceinline__ void stage_mixed_body(const StageParams& P, const Ops
         elem_sync<NP>();
         if constexpr (LEARN) {
             if (P.mlp) {   // learned sensor: wave w evaluates elements 16 w .. 16 w + 15
-                // its scratch is the face-flux array the other waves' row passes read
-                __syncthreads();
+                // scratch: the trace array, dead since the flux barrier (LEARN implies SEPF)
                 const int w = tid >> 6, lane = tid & 63;
                 const mlp_f4 o = mlp_wave16(P.mlp, sU + (16 * w + (lane & 15)) * ESTR + kModes,
                                             float(P.eps_floor), reinterpret_cast<short*>(sXW) + w * 512);

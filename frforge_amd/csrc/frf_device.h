@@ -165,6 +165,22 @@ __device__ __forceinline__ float persson_sigma_hw(float eh, float et, float s0, 
     return __builtin_amdgcn_rcpf(1.0f + ex);
 }
 
+// Hand-over of LDS data that only an element's own Np line threads read.  When Np divides the
+// wave size an element's lanes are consecutive lanes of one wave, whose LDS instructions execute
+// in order: the wavefront-scope fences (which keep the compiler from moving accesses across) are
+// enough, and waves do not wait for each other.  Otherwise elements straddle waves and the
+// workgroup barrier stays.
+template <int NP>
+__device__ __forceinline__ void elem_sync() {
+    if constexpr (64 % NP == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Boundary time of a launch: the state time of a residual, and the reference's stage times t0,
 // t0 + dt, t0 + dt/2 inside an SSP-RK3 step (SSP_RK3.jl:40-60).
 __device__ __forceinline__ double boundary_time(const StageParams& P) {

@@ -62,21 +62,6 @@
 
 namespace frf {
 
-// Hand-over of LDS data that only an element's own Np line threads read.  When Np divides the
-// wave size an element's lanes are consecutive lanes of one wave, whose LDS instructions execute
-// in order: the wavefront-scope fences (which keep the compiler from moving accesses across) are
-// enough, and waves do not wait for each other.  Otherwise elements straddle waves and the
-// workgroup barrier stays.
-template <int NP>
-__device__ __forceinline__ void elem_sync() {
-    if constexpr (64 % NP == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
-}
 
 // ---- Line operators that are exact on constant lines.
 // With d_k = v_k - v_0 a Lagrange trace (weights summing to one) is v_0 + sum_{k>=1} w_k d_k

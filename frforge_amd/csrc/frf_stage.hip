@@ -215,7 +215,9 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             }
         }
     }
-    __syncthreads();
+    // P1 reads only this element's slot and viscosity; the halo lines are read in P2, after the
+    // barrier that follows P1
+    elem_sync<NP>();
 
     // ------------------------------------------------------------------ P1: line sweeps
     const double eps_e = AV ? sEpsT[le] : 0.0;
@@ -651,7 +653,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             }
         }
     }
-    __syncthreads();
+    elem_sync<NP>();   // the row pass reads this element's slot (neighbours' fluxes: barrier above)
 
     bool bad = false, bad_u = false;
     double lam_row = 0.0;
@@ -725,15 +727,15 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
             }
         }
     }
-    if (P.stage > 0) {
-        const int any_bad = __syncthreads_or(bad ? 1 : 0);
-        const int any_bad_u = (P.stage == 3) ? __syncthreads_or(bad_u ? 1 : 0) : 0;
-        if (tid == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
+    if (P.stage > 0) {   // per wave: the minimum over waves is the block's (stage < 4)
+        const bool any_bad = __ballot(bad) != 0;
+        const bool any_bad_u = P.stage == 3 && __ballot(bad_u) != 0;
+        if ((tid & 63) == 0 && (any_bad || any_bad_u)) atomicMin(&P.ctl->fail_stage, any_bad ? P.stage : 4);
     }
 
     // ------------------------------------------------------------------ P4: epilogue
     if (P.stage == 0 || !(P.want_lambda || AV)) return;
-    __syncthreads();
+    elem_sync<NP>();   // the row reads of this element's slot are done: it becomes scratch
     R* sT = sU;   // modal transform scratch: [le][a][j]
     if (valid && AV) {
         const int j = t;
@@ -746,7 +748,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
         }
     }
     sRed[tid] = lam_row;
-    __syncthreads();
+    elem_sync<NP>();
     if (valid && AV) {
         const int b = t;
         R et = R(0), eh = R(0), ecol = R(0);
@@ -763,7 +765,7 @@ __global__ void __launch_bounds__(TX* TY* NP, (CAPT == 0 && sizeof(R) == 8) ? 2 
         sRed[NTHR + tid] = double(et);
         sRed[2 * NTHR + tid] = double(eh);
     }
-    __syncthreads();
+    elem_sync<NP>();
     if (valid && t == 0) {
         double lm = 0.0;
 #pragma unroll

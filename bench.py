@@ -37,7 +37,7 @@ CFL = 0.08
 # HBM bytes per FP32-mixed Persson+BR0 stage-kernel launch on this workload, from rocprofv3
 # --pmc FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors a 1 GiB device-to-device copy
 # gives; profiles/stage_kernel_1024x1024_p3_riemann.md holds the run.
-MEASURED_TRAFFIC = 1505.9e6
+MEASURED_TRAFFIC = 1509.8e6
 
 
 def parse():

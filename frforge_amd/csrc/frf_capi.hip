@@ -254,6 +254,7 @@ extern "C" int32_t frf_set_learned_sensor(frf_ctx* c, const void* packed, int64_
         return fail(c, FRF_E_UNSUPPORTED, "the learned sensor runs in the FP32-mixed form");
     if (c->pb.capturing != FRF_CAPT_PERSSON_BR0 && c->pb.capturing != FRF_CAPT_PERSSON_DD)
         return fail(c, FRF_E_UNSUPPORTED, "the learned sensor replaces the Persson sigma of a Persson AV method");
+    if (c->dMet) return fail(c, FRF_E_UNSUPPORTED, "the learned sensor runs in the affine FP32-mixed kernel");
     if (!c->dMlp) HIPCHK(c, hipMalloc(&c->dMlp, sizeof(frf::MlpPacked)));
     HIPCHK(c, hipMemcpyAsync(c->dMlp, packed, sizeof(frf::MlpPacked), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -293,8 +294,8 @@ extern "C" int32_t frf_set_curved_metrics(frf_ctx* c, const double* x_xi, const 
     }
     for (auto* a : vol) if (!a) return fail(c, FRF_E_ARG, "frf_set_curved_metrics: null array");
     for (auto* a : face) if (!a) return fail(c, FRF_E_ARG, "frf_set_curved_metrics: null array");
-    if (c->pb.precision != FRF_FP64)
-        return fail(c, FRF_E_UNSUPPORTED, "curved metrics run in the FP64 form (the FP32-mixed form is direction-separable)");
+    if (c->dMlp)
+        return fail(c, FRF_E_UNSUPPORTED, "the learned sensor runs in the affine FP32-mixed kernel");
     const int32_t sides[4] = {c->pb.bc_left, c->pb.bc_right, c->pb.bc_bottom, c->pb.bc_top};
     for (int k = 0; k < 4; ++k)
         if (sides[k] == FRF_BC_GHOST_SPLIT)
@@ -911,6 +912,10 @@ extern "C" int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch**
         }
         if (c->pb.precision != FRF_FP32_MIXED) {
             g_create_err = "frf_batch_create: the batched form runs the FP32-mixed form only";
+            return FRF_E_UNSUPPORTED;
+        }
+        if (c->dMet) {   // the group kernels take the affine tables
+            g_create_err = "frf_batch_create: instance " + std::to_string(i) + " has curved metrics (single contexts only)";
             return FRF_E_UNSUPPORTED;
         }
         if (c->dSolid) {   // the group kernels are built without the solid-mask face logic

@@ -860,9 +860,48 @@ static hipError_t launch_stage_fc(int flux, int capt, const StageParams& P, cons
     return launch_stage_t<NP, R, 1, 2>(P, h, s);
 }
 
+// FP32 arithmetic on curved metrics: this kernel's general-metric variant in float (FP64 state
+// and stage combine), since the direction-separable mixed kernel needs x_eta = y_xi = 0.
+template <int NP, int FLUX, int CAPT>
+static hipError_t launch_curved_f32_t(const StageParams& P0, const HostOps& h, hipStream_t s) {
+    constexpr int TX = Tile<NP>::TX, TY = Tile<NP>::TY;
+    StageParams P = P0;
+    P.nblk = ((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY);
+    constexpr bool PF = FRF_PREFETCH_U0 < 0 ? (CAPT != 0) : (FRF_PREFETCH_U0 != 0);
+    hipLaunchKernelGGL((stage_kernel<NP, float, FLUX, CAPT, TX, TY, PF, true>), dim3(P.nblk), dim3(TX * TY * NP),
+                       0, s, P, make_ops<NP, float>(h));
+    return hipGetLastError();
+}
+
+template <int NP>
+static hipError_t launch_curved_f32_np(int flux, int capt, const StageParams& P, const HostOps& h,
+                                       hipStream_t s) {
+    if (flux == 0) {
+        if (capt == 0) return launch_curved_f32_t<NP, 0, 0>(P, h, s);
+        if (capt == 1) return launch_curved_f32_t<NP, 0, 1>(P, h, s);
+        return launch_curved_f32_t<NP, 0, 2>(P, h, s);
+    }
+    if (capt == 0) return launch_curved_f32_t<NP, 1, 0>(P, h, s);
+    if (capt == 1) return launch_curved_f32_t<NP, 1, 1>(P, h, s);
+    return launch_curved_f32_t<NP, 1, 2>(P, h, s);
+}
+
+static hipError_t launch_curved_f32(int np, int flux, int capt, const StageParams& P, const HostOps& h,
+                                    hipStream_t s) {
+    switch (np) {
+        case 2: return launch_curved_f32_np<2>(flux, capt, P, h, s);
+        case 3: return launch_curved_f32_np<3>(flux, capt, P, h, s);
+        case 4: return launch_curved_f32_np<4>(flux, capt, P, h, s);
+        case 5: return launch_curved_f32_np<5>(flux, capt, P, h, s);
+        case 6: return launch_curved_f32_np<6>(flux, capt, P, h, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_stage(int np, int prec, int flux, int capt, const StageParams& P,
                         const HostOps& h, hipStream_t s) {
-    if (prec != 0) return launch_stage_mixed(np, flux, capt, P, h, s);
+    if (prec != 0)
+        return P.met ? launch_curved_f32(np, flux, capt, P, h, s) : launch_stage_mixed(np, flux, capt, P, h, s);
     switch (np) {
         case 2: return launch_stage_fc<2, double>(flux, capt, P, h, s);
         case 3: return launch_stage_fc<3, double>(flux, capt, P, h, s);

@@ -92,8 +92,10 @@ int32_t frf_set_boundary_state(frf_ctx* ctx, int32_t side, const double* U);
  * (:320-442): x_xi, x_eta, y_xi, y_eta, J as (Np,Np,Nel) and the unit normals and scalings of the
  * four faces as (Np,Nel), in the reference's layout.  The residual then takes its volume terms
  * and face normals from them instead of the affine tables; BR0, local DD, the element size and dt
- * keep the affine Jx, Jy and min(dx, dy), as the reference's do for analytic wavy metrics.  FP64
- * form only, not with split sides; x_xi = NULL returns to the affine metrics. */
+ * keep the affine Jx, Jy and min(dx, dy), as the reference's do for analytic wavy metrics.  In
+ * the FP32-mixed form a general-metric FP32 kernel (FP64 state and stage combine) stands in for
+ * the direction-separable one.  Not with split sides, the learned sensor or a batch; x_xi = NULL
+ * returns to the affine metrics. */
 int32_t frf_set_curved_metrics(frf_ctx* ctx, const double* x_xi, const double* x_eta,
                                const double* y_xi, const double* y_eta, const double* J,
                                const double* nx_W, const double* ny_W, const double* sJ_W,

@@ -1,7 +1,7 @@
-"""Curved metrics on the GPU (FP64 form) against the oracle given the same metric arrays (whose
-curved path tests/test_oracle_curved.py pins): per-stage residuals on wavy meshes within 1e-12,
-with periodic, transmissive and reflecting sides and a solid block; the curved cases run step for
-step; and the checks of frf_set_curved_metrics."""
+"""Curved metrics on the GPU against the oracle given the same metric arrays (whose curved path
+tests/test_oracle_curved.py pins): per-stage residuals on wavy meshes within 1e-12 (FP64) and 1e-4
+(FP32-mixed), with periodic, transmissive and reflecting sides and a solid block; the curved cases
+run step for step; and the checks of frf_set_curved_metrics."""
 import numpy as np
 import pytest
 
@@ -106,16 +106,56 @@ def test_curved_freestream_and_suite_on_the_device():
     assert overall and hard == []
 
 
+@pytest.mark.parametrize("capturing", ["null", "persson_br0", "persson_dd"])
+@pytest.mark.parametrize("sides", ["periodic", "walls"])
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_curved_residual_mixed(p, sides, capturing):
+    """The FP32-mixed form on curved metrics (the general-metric kernel in FP32, FP64 state):
+    within 1e-4 of the oracle normwise."""
+    ops = build_operators(p)
+    st_ref, m, orc = _case(p, capturing, "rusanov", sides, noise=0.05)
+    st = allocate_state(st_ref.mesh, ops, precision="fp32_mixed", wavy_amp=0.05)
+    st.u[...] = st_ref.u
+    du = np.zeros(st.u.shape, order="F")
+    residual(du, st, EQ, m)
+    err = rel_err_per_component(du, orc.residual(st.u))
+    print(f"curved mixed p={p} {sides} {capturing}: {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
+def test_curved_mixed_run_follows_fp64():
+    st, m, _ = _case(3, "persson_br0", "rusanov", "periodic")
+    st_m = allocate_state(st.mesh, st.ops, precision="fp32_mixed", wavy_amp=0.05)
+    st_m.u[...] = st.u
+    r = ssp_rk3(st, EQ, m, 0.03, cfl=0.1)
+    r_m = ssp_rk3(st_m, EQ, m, 0.03, cfl=0.1)
+    err = rel_err_per_component(st_m.u, st.u)
+    print(f"curved mixed run: n={r_m['n_steps']} (fp64 {r['n_steps']}) rel err {err:.3e}", flush=True)
+    assert (r_m["status"], r_m["n_steps"]) == (r["status"], r["n_steps"])
+    assert err <= 1e-5
+
+
 def test_curved_metric_checks():
     from frforge_amd.mesh import build_mesh_metrics
-    # the FP32-mixed form is direction-separable: curved metrics are refused
     st, m, _ = _case(2, "null", "rusanov")
-    st_m = allocate_state(st.mesh, st.ops, precision="fp32_mixed", wavy_amp=0.05)
-    set_initial_condition(st_m, _smooth)
-    with pytest.raises(FrfError, match="FP64 form"):
-        residual(np.zeros(st_m.u.shape, order="F"), st_m, EQ, m)
     # discrete metrics of a wavy mesh carry a curved h_char the device does not keep
     st_d = allocate_state(st.mesh, st.ops, metrics=build_mesh_metrics(st.mesh, st.ops))
     set_initial_condition(st_d, _smooth)
     with pytest.raises(NotImplementedError, match="h_char"):
         residual(np.zeros(st_d.u.shape, order="F"), st_d, EQ, m)
+    # the learned sensor lives in the affine mixed kernel
+    from frforge_amd.device import DeviceContext
+    dev = DeviceContext(p=3, nx=4, ny=4, bc_left="periodic", bc_right="periodic", bc_bottom="periodic",
+                        bc_top="periodic", precision="fp32_mixed", capturing="persson_br0")
+    try:
+        ops = build_operators(3)
+        dev.set_operators(ops.xi, ops.w, ops.D, ops.ell_L, ops.ell_R, ops.gL_xi, ops.gR_xi, ops.V_legendre)
+        mesh = make_wavy_mesh2d(4, 4, ops, amp=0.05)
+        dev.set_geometry(mesh.x_vertices, mesh.y_vertices)
+        st_w = allocate_state(mesh, ops, wavy_amp=0.05)
+        dev.set_curved_metrics(st_w.metrics)
+        from frforge_amd.learned import MLPWeights
+        with pytest.raises(FrfError, match="affine FP32-mixed kernel"):
+            dev.set_learned_sensor(MLPWeights.init(seed=0).pack())
+    finally:
+        dev.close()

@@ -49,6 +49,8 @@ struct Par1 {
     const Ops1* op;
     Ctl1* ctl;
     int np, nel, periodic, capt;
+    int dir_l, dir_r;          // Dirichlet ends (constant ghost states ud_l, ud_r)
+    double ud_l[3], ud_r[3];
     double gamma, kappa, s0, eps_floor, c_av;
 };
 
@@ -122,10 +124,10 @@ __device__ void residual(const Par1& P, const double* __restrict__ u) {
             for (int c = 0; c < 3; ++c) { um[c] = EE(P.uR, i - 1, c); up[c] = EE(P.uL, i, c); }
             rusanov(g, um, up, f);
             for (int c = 0; c < 3; ++c) { EE(P.fR, i - 1, c) = f[c]; EE(P.fL, i, c) = f[c]; }
-        } else if (i == 0) {
+        } else if (i == 0) {   // Residual.jl:63-90: periodic, transmissive or Dirichlet ghost
             for (int c = 0; c < 3; ++c) {
                 up[c] = EE(P.uL, 0, c);
-                um[c] = P.periodic ? EE(P.uR, Nel - 1, c) : EE(P.uL, 0, c);
+                um[c] = P.periodic ? EE(P.uR, Nel - 1, c) : (P.dir_l ? P.ud_l[c] : EE(P.uL, 0, c));
             }
             rusanov(g, um, up, f);
             for (int c = 0; c < 3; ++c) {
@@ -133,7 +135,10 @@ __device__ void residual(const Par1& P, const double* __restrict__ u) {
                 if (P.periodic) EE(P.fR, Nel - 1, c) = f[c];
             }
         } else if (!P.periodic) {   // i == Nel
-            for (int c = 0; c < 3; ++c) { um[c] = EE(P.uR, Nel - 1, c); up[c] = EE(P.uR, Nel - 1, c); }
+            for (int c = 0; c < 3; ++c) {
+                um[c] = EE(P.uR, Nel - 1, c);
+                up[c] = P.dir_r ? P.ud_r[c] : EE(P.uR, Nel - 1, c);
+            }
             rusanov(g, um, up, f);
             for (int c = 0; c < 3; ++c) EE(P.fR, Nel - 1, c) = f[c];
         }
@@ -428,6 +433,7 @@ struct frf1d_ctx {
     Par1 P{};
     Ops1 ops{};
     bool ops_set = false, geom_set = false, state_set = false;
+    bool bc_set[2] = {false, false};   // Dirichlet ends with a state
 };
 
 static thread_local std::string g_err1;
@@ -472,7 +478,8 @@ extern "C" int32_t frf1d_create(const frf1d_problem* pb, frf1d_ctx** out) {
     if ((pb->bc_left == FRF_BC_PERIODIC) != (pb->bc_right == FRF_BC_PERIODIC))
         return fail1(nullptr, FRF_E_ARG, "Periodic BC must be set on both ends");
     for (int32_t b : {pb->bc_left, pb->bc_right})
-        if (b != FRF_BC_PERIODIC && b != FRF_BC_TRANSMISSIVE) return fail1(nullptr, FRF_E_UNSUPPORTED, "unsupported BC %d", b);
+        if (b != FRF_BC_PERIODIC && b != FRF_BC_TRANSMISSIVE && b != FRF_BC_DIRICHLET)
+            return fail1(nullptr, FRF_E_UNSUPPORTED, "unsupported BC %d", b);
     if (pb->flux != FRF_FLUX_RUSANOV) return fail1(nullptr, FRF_E_UNSUPPORTED, "the 1D device path implements the Rusanov flux");
     if (pb->precision != FRF_FP64) return fail1(nullptr, FRF_E_UNSUPPORTED, "the 1D device path runs in FP64");
     if (pb->capturing < FRF_CAPT_NULL || pb->capturing > FRF_CAPT_PERSSON_DD)
@@ -514,6 +521,8 @@ extern "C" int32_t frf1d_create(const frf1d_problem* pb, frf1d_ctx** out) {
     P.np = c->np;
     P.nel = c->nel;
     P.periodic = pb->bc_left == FRF_BC_PERIODIC;
+    P.dir_l = pb->bc_left == FRF_BC_DIRICHLET;
+    P.dir_r = pb->bc_right == FRF_BC_DIRICHLET;
     P.capt = pb->capturing;
     P.gamma = pb->gamma;
     P.kappa = pb->kappa;
@@ -521,6 +530,18 @@ extern "C" int32_t frf1d_create(const frf1d_problem* pb, frf1d_ctx** out) {
     P.eps_floor = pb->eps_floor;
     P.c_av = pb->c_av;
     *out = c;
+    return FRF_OK;
+}
+
+extern "C" int32_t frf1d_set_boundary_state(frf1d_ctx* c, int32_t side, const double* U) {
+    if (!c) return FRF_E_ARG;
+    if (side < 0 || side > 1 || !U) return fail1(c, FRF_E_ARG, "frf1d_set_boundary_state: bad side or state");
+    if (!(side == 0 ? c->P.dir_l : c->P.dir_r)) return fail1(c, FRF_E_ARG, "end %d is not a Dirichlet end", side);
+    for (int q = 0; q < 3; ++q) {
+        if (!std::isfinite(U[q])) return fail1(c, FRF_E_ARG, "non-finite Dirichlet state");
+        (side == 0 ? c->P.ud_l : c->P.ud_r)[q] = U[q];
+    }
+    c->bc_set[side] = true;
     return FRF_OK;
 }
 
@@ -601,6 +622,8 @@ static int32_t ready1(frf1d_ctx* c) {
     if (!c->ops_set) return fail1(c, FRF_E_STATE, "operators not set");
     if (!c->geom_set) return fail1(c, FRF_E_STATE, "geometry not set");
     if (!c->state_set) return fail1(c, FRF_E_STATE, "no state uploaded");
+    if ((c->P.dir_l && !c->bc_set[0]) || (c->P.dir_r && !c->bc_set[1]))
+        return fail1(c, FRF_E_STATE, "a Dirichlet end has no state (frf1d_set_boundary_state)");
     return FRF_OK;
 }
 

@@ -32,7 +32,8 @@ EXPORTED = ("frf_create", "frf_set_operators", "frf_set_geometry", "frf_set_boun
             "frf_device_synchronize", "frf_stream", "frf_last_error", "frf_destroy",
             "frf_batch_create", "frf_batch_tune", "frf_batch_run", "frf_batch_tile",
             "frf_batch_last_error", "frf_batch_destroy",
-            "frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_upload_state",
+            "frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_set_boundary_state",
+            "frf1d_upload_state",
             "frf1d_download_state", "frf1d_residual", "frf1d_rk3_step", "frf1d_run",
             "frf1d_compute_dt", "frf1d_last_error", "frf1d_destroy")
 
@@ -122,6 +123,7 @@ def load_library(path: str = LIB_PATH):
     L.frf1d_create.argtypes = [C.POINTER(Frf1dProblem), C.POINTER(vp)]
     L.frf1d_set_operators.argtypes = [vp] + [dp] * 8
     L.frf1d_set_geometry.argtypes = [vp, dp]
+    L.frf1d_set_boundary_state.argtypes = [vp, C.c_int32, dp]
     L.frf1d_upload_state.argtypes = [vp, dp, C.c_int64, C.c_double]
     L.frf1d_download_state.argtypes = [vp, dp, C.c_int64, dp]
     L.frf1d_residual.argtypes = [vp, dp, C.c_int64]
@@ -131,7 +133,8 @@ def load_library(path: str = LIB_PATH):
     L.frf1d_last_error.restype = C.c_char_p
     L.frf1d_last_error.argtypes = [vp]
     L.frf1d_destroy.argtypes = [vp]
-    for name in ("frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_upload_state",
+    for name in ("frf1d_create", "frf1d_set_operators", "frf1d_set_geometry", "frf1d_set_boundary_state",
+                 "frf1d_upload_state",
                  "frf1d_download_state", "frf1d_residual", "frf1d_rk3_step", "frf1d_run",
                  "frf1d_compute_dt", "frf1d_destroy"):
         getattr(L, name).restype = C.c_int32
@@ -375,9 +378,10 @@ class DeviceContext1D:
     """frf1d_ctx: a 1D Euler problem (Euler1D, FP64) resident on the GPU."""
 
     def __init__(self, *, p, nel, bc="transmissive", capturing="null", gamma=1.4, kappa=4.0,
-                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1):
+                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1, bc_right=None):
+        """bc: the kind of both ends, or of the left end when bc_right is given."""
         self.lib = load_library()
-        pb = Frf1dProblem(p, nel, 3, BC[bc], BC[bc], FLUX["rusanov"], PRECISION["fp64"],
+        pb = Frf1dProblem(p, nel, 3, BC[bc], BC[bc_right or bc], FLUX["rusanov"], PRECISION["fp64"],
                           CAPTURING[capturing], gamma, kappa, s0_factor, eps_floor, c_av)
         h = C.c_void_p()
         rc = self.lib.frf1d_create(C.byref(pb), C.byref(h))
@@ -410,6 +414,14 @@ class DeviceContext1D:
     def set_geometry(self, x_vertices):
         xv = np.ascontiguousarray(x_vertices, dtype=np.float64)
         self._chk(self.lib.frf1d_set_geometry(self.h, _dp(xv)), "frf1d_set_geometry")
+
+    def set_boundary_state(self, side: str, U):
+        """Ghost state (3 conserved values) of a Dirichlet end, side "left" or "right"."""
+        Ua = np.ascontiguousarray(U, dtype=np.float64)
+        if Ua.shape != (3,):
+            raise ValueError("a 1D Dirichlet state has 3 conserved components")
+        self._chk(self.lib.frf1d_set_boundary_state(self.h, {"left": 0, "right": 1}[side], _dp(Ua)),
+                  "frf1d_set_boundary_state")
 
     def _check(self, u):
         if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.f_contiguous

@@ -10,7 +10,7 @@ from typing import Any, Optional
 
 import numpy as np
 
-from .mesh import PeriodicBC, TransmissiveBC
+from .mesh import DirichletBC, PeriodicBC, TransmissiveBC
 from .operators import FROperators
 from .state import SchemeConfig
 
@@ -70,12 +70,22 @@ class Mesh1D:
         xL, xR = self.x_vertices[:-1], self.x_vertices[1:]
         return (xL + xR)[None, :] / 2.0 + self.J[None, :] * ops.xi[:, None]
 
-    def bc_name(self) -> str:
-        if isinstance(self.left_bc, PeriodicBC):
-            return "periodic"
-        if isinstance(self.left_bc, TransmissiveBC) and isinstance(self.right_bc, TransmissiveBC):
-            return "transmissive"
-        raise NotImplementedError("the 1D device path implements periodic and transmissive ends")
+    def bc_names(self):
+        """Device kinds of the (left, right) ends: periodic, transmissive, or a constant Dirichlet
+        state (``DirichletBC(state=U)``, Residual.jl:75-79,100-104)."""
+        def one(bc):
+            if isinstance(bc, PeriodicBC):
+                return "periodic"
+            if isinstance(bc, TransmissiveBC):
+                return "transmissive"
+            if isinstance(bc, DirichletBC):
+                if bc.state is None:
+                    raise NotImplementedError("the 1D device path takes constant Dirichlet states "
+                                              "(DirichletBC(state=U))")
+                return "dirichlet"
+            raise NotImplementedError("the 1D device path implements periodic, transmissive and "
+                                      "Dirichlet ends")
+        return one(self.left_bc), one(self.right_bc)
 
 
 @dataclass

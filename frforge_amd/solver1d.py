@@ -16,18 +16,24 @@ def _device1d(state: SolutionState1D, eq: Euler1D, method: AbstractCapturingMeth
         raise NotImplementedError("the 1D device path implements the Rusanov flux")
     kind, prm = method.device_capturing()
     mesh = state.mesh
-    key = (state.p, mesh.n_elements, tuple(mesh.x_vertices), mesh.bc_name(), kind,
+    ends = mesh.bc_names()
+    states = tuple(b.state if ends[k] == "dirichlet" else None
+                   for k, b in enumerate((mesh.left_bc, mesh.right_bc)))
+    key = (state.p, mesh.n_elements, tuple(mesh.x_vertices), ends, states, kind,
            tuple(sorted(prm.items())), float(eq.gamma), id(state.ops))
     dev = state.device
     if dev is not None and getattr(dev, "key", None) == key:
         return dev
     if dev is not None:
         dev.close()
-    dev = DeviceContext1D(p=state.p, nel=mesh.n_elements, bc=mesh.bc_name(), capturing=kind,
+    dev = DeviceContext1D(p=state.p, nel=mesh.n_elements, bc=ends[0], bc_right=ends[1], capturing=kind,
                           gamma=float(eq.gamma), **prm)
     ops = state.ops
     dev.set_operators(ops.xi, ops.w, ops.D, ops.ell_L, ops.ell_R, ops.gL_xi, ops.gR_xi, ops.V_legendre)
     dev.set_geometry(mesh.x_vertices)
+    for side, U in zip(("left", "right"), states):
+        if U is not None:
+            dev.set_boundary_state(side, U)
     dev.key = key
     state.device = dev
     return dev

@@ -212,6 +212,10 @@ int32_t frf1d_set_operators(frf1d_ctx* ctx, const double* xi, const double* w, c
                             const double* gR_xi, const double* V_legendre);
 /* Mesh1D.x_vertices (nel+1 values; Mesh1D.jl:35-48) */
 int32_t frf1d_set_geometry(frf1d_ctx* ctx, const double* x_vertices);
+/* The ghost state U[3] of a FRF_BC_DIRICHLET end (side 0 left, 1 right): DirichletBC with a
+ * constant u_func (Residual.jl:75-79,100-104); BR0 treats the end as transmissive
+ * (PerssonAV.jl:321-333).  Required before a run for every Dirichlet end. */
+int32_t frf1d_set_boundary_state(frf1d_ctx* ctx, int32_t side, const double* U);
 int32_t frf1d_upload_state(frf1d_ctx* ctx, const double* u, int64_t len, double t);
 int32_t frf1d_download_state(frf1d_ctx* ctx, double* u, int64_t len, double* t);
 /* residual!(du, state, eq, method) 1D (Residual.jl:161-214) */

@@ -1177,6 +1177,7 @@ int orc_run(orc_ctx* h, double* u, double* t, double t_final, double cfl, double
 
 struct orc1_ctx {
     orc1_problem pb;
+    double bc_state[2][3];            /* Dirichlet ends: the ghost states (left, right) */
     int Np, Nel;
     double *xi, *w, *D, *lL, *lR, *gLxi, *gRxi, *V;
     double *xv, *dx, *J;
@@ -1214,6 +1215,15 @@ static void rusanov1(double g, const double* uL, const double* uR, double* out) 
     double lam = jmax(fabs(v1(uL[0], uL[1])) + c1(g, uL[0], uL[1], uL[2]),
                       fabs(v1(uR[0], uR[1])) + c1(g, uR[0], uR[1], uR[2]));
     for (int c = 0; c < 3; ++c) out[c] = 0.5 * (fL[c] + fR[c]) - 0.5 * lam * (uR[c] - uL[c]);
+}
+
+/* A constant DirichletBC end (Residual.jl:75-79,100-104): the ghost state U[3]; side 0 left,
+ * 1 right.  BR0 treats the end as transmissive (PerssonAV.jl:321-333). */
+int orc1_set_boundary_state(orc1_ctx* h, int side, const double* U) {
+    if (side < 0 || side > 1 || !U) return -1;
+    if ((side == 0 ? h->pb.bc_left : h->pb.bc_right) != ORC_BC_DIRICHLET) return -1;
+    memcpy(h->bc_state[side], U, sizeof h->bc_state[side]);
+    return 0;
 }
 
 orc1_ctx* orc1_create(const orc1_problem* pb, const double* xi, const double* w, const double* D,
@@ -1317,9 +1327,10 @@ void orc1_residual(orc1_ctx* h, const double* u, double t, double* du) {
         for (int c = 0; c < 3; ++c) { E1(h->fR, e, c) = f[c]; E1(h->fL, e + 1, c) = f[c]; }
     }
     const int periodic = h->pb.bc_left == ORC_BC_PERIODIC;
-    for (int c = 0; c < 3; ++c) {
+    const int dir_l = h->pb.bc_left == ORC_BC_DIRICHLET, dir_r = h->pb.bc_right == ORC_BC_DIRICHLET;
+    for (int c = 0; c < 3; ++c) {   /* Residual.jl:63-111: u_m of the left end, u_p of the right */
         up[c] = E1(h->uL, 0, c);
-        um[c] = periodic ? E1(h->uR, Nel - 1, c) : E1(h->uL, 0, c);
+        um[c] = periodic ? E1(h->uR, Nel - 1, c) : (dir_l ? h->bc_state[0][c] : E1(h->uL, 0, c));
     }
     rusanov1(g, um, up, f);
     for (int c = 0; c < 3; ++c) {
@@ -1327,7 +1338,10 @@ void orc1_residual(orc1_ctx* h, const double* u, double t, double* du) {
         if (periodic) E1(h->fR, Nel - 1, c) = f[c];
     }
     if (!periodic) {
-        for (int c = 0; c < 3; ++c) { um[c] = E1(h->uR, Nel - 1, c); up[c] = E1(h->uR, Nel - 1, c); }
+        for (int c = 0; c < 3; ++c) {
+            um[c] = E1(h->uR, Nel - 1, c);
+            up[c] = dir_r ? h->bc_state[1][c] : E1(h->uR, Nel - 1, c);
+        }
         rusanov1(g, um, up, f);
         for (int c = 0; c < 3; ++c) E1(h->fR, Nel - 1, c) = f[c];
     }

@@ -116,6 +116,8 @@ orc1_ctx* orc1_create(const orc1_problem* pb, const double* xi, const double* w,
                       const double* lL, const double* lR, const double* gLxi, const double* gRxi,
                       const double* V, const double* xv);
 void orc1_destroy(orc1_ctx* h);
+/* The ghost state U[3] of a constant Dirichlet end (bc ORC_BC_DIRICHLET; side 0 left, 1 right). */
+int orc1_set_boundary_state(orc1_ctx* h, int side, const double* U);
 void orc1_residual(orc1_ctx* h, const double* u, double t, double* du);   /* Residual.jl:161-214 */
 void orc1_sense(orc1_ctx* h, const double* u, double* sigma);             /* PerssonAV.jl:116-135 */
 void orc1_element_viscosities(orc1_ctx* h, const double* u, const double* sigma, double* eps);

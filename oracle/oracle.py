@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libfrf_oracle.so")
 
-BC_PERIODIC, BC_TRANSMISSIVE = 0, 1
+BC_PERIODIC, BC_TRANSMISSIVE, BC_DIRICHLET = 0, 1, 3
 FLUX = {"rusanov": 0, "hllc": 1}
 CAPT = {"null": 0, "persson_br0": 1, "persson_dd": 2}
 STATUS = {0: "ok", 1: "diverged", 2: "max_steps"}
@@ -62,6 +62,8 @@ def lib():
         L.orc_residual.argtypes = [C.c_void_p, dp, C.c_double, dp]
         L.orc_set_solid.restype = C.c_int
         L.orc_set_solid.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        L.orc1_set_boundary_state.restype = C.c_int
+        L.orc1_set_boundary_state.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_set_boundary.restype = C.c_int
         L.orc_set_boundary.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_set_metrics.restype = C.c_int
@@ -269,9 +271,15 @@ class Oracle1D:
     """The 1D Euler restatement (Sod tube configuration): u[j, e, c] with shape (Np, Nel, 3)."""
 
     def __init__(self, p, nel, xv, ops, bc="transmissive", capturing="null", gamma=1.4, kappa=4.0,
-                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1):
+                 s0_factor=-4.0, eps_floor=1e-16, c_av=0.1, dirichlet=None):
+        """dirichlet: optional (U_left, U_right) constant ghost states (None for a transmissive
+        end) on a non-periodic mesh."""
         b = BC_PERIODIC if bc == "periodic" else BC_TRANSMISSIVE
-        pb = Orc1Problem(p, nel, b, b, CAPT[capturing], 0, gamma, kappa, s0_factor, eps_floor, c_av)
+        ends = [b, b]
+        for k, U in enumerate(dirichlet or (None, None)):
+            if U is not None:
+                ends[k] = BC_DIRICHLET
+        pb = Orc1Problem(p, nel, ends[0], ends[1], CAPT[capturing], 0, gamma, kappa, s0_factor, eps_floor, c_av)
         self.p, self.nel, self.Np = p, nel, p + 1
         self._keep = {k: np.asfortranarray(np.asarray(ops[k], dtype=np.float64))
                       for k in ("xi", "w", "D", "lL", "lR", "gLxi", "gRxi", "V")}
@@ -281,6 +289,13 @@ class Oracle1D:
                                    _p(k["lR"]), _p(k["gLxi"]), _p(k["gRxi"]), _p(k["V"]), _p(self._xv))
         if not self.h:
             raise ValueError("orc1_create failed")
+        self._bc_states = []
+        for k, U in enumerate(dirichlet or (None, None)):
+            if U is not None:
+                Ua = np.ascontiguousarray(U, dtype=np.float64)
+                self._bc_states.append(Ua)
+                if lib().orc1_set_boundary_state(self.h, k, _p(Ua)) != 0:
+                    raise ValueError("orc1_set_boundary_state failed")
 
     def __del__(self):
         if getattr(self, "h", None):

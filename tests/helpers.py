@@ -99,3 +99,22 @@ def oracle_integrator_2d(state, eq, method, t_final, cfl, dt=None):
     state.u[...] = u
     state.t = t
     return {"status": st, "n_steps": n, "t": t}
+
+
+def oracle_for_state_1d(state, method, eq):
+    """The 1D oracle problem matching a 1D state (ends, Dirichlet states) and a method."""
+    from oracle.oracle import Oracle1D
+    kind, prm = method.device_capturing()
+    m = state.mesh
+    bc = "periodic" if isinstance(m.left_bc, PeriodicBC) else "transmissive"
+    dirichlet = tuple(b.state if isinstance(b, DirichletBC) else None for b in (m.left_bc, m.right_bc))
+    return Oracle1D(state.p, m.n_elements, m.x_vertices, oracle_ops(state.ops), bc=bc, capturing=kind,
+                    gamma=eq.gamma, dirichlet=dirichlet, **prm)
+
+
+def oracle_integrator_1d(state, eq, method, t_final, cfl, dt=None):
+    """ssp_rk3! (1D) through the oracle's time loop, writing the result back into the state."""
+    st, n, t, u = oracle_for_state_1d(state, method, eq).run(state.u, state.t, t_final, cfl=cfl, dt=dt)
+    state.u[...] = u
+    state.t = t
+    return {"status": st, "n_steps": n, "t": t}

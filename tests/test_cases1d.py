@@ -1,0 +1,70 @@
+"""1D Dirichlet ends (Residual.jl:63-111, PerssonAV.jl:321-333) in the oracle, pinned without the
+reference's outputs: a Dirichlet end whose state equals the interior end trace reproduces the
+transmissive residual bit for bit, any other state changes only the end elements; and the
+reference's BC freestream cases (test/test_euler.jl:55-63) through the oracle's time loop."""
+import numpy as np
+import pytest
+
+from frforge_amd import NullCapturing, PerssonAVMethod, build_operators
+from frforge_amd.cases1d import run_bc_dirichlet_test, run_bc_transmissive_test
+from frforge_amd.mesh import DirichletBC, TransmissiveBC
+from frforge_amd.oned import Euler1D, Mesh1D, allocate_state_1d, set_initial_condition_1d, sod_ic
+
+from helpers import oracle_for_state_1d, oracle_integrator_1d
+
+EQ1 = Euler1D(1.4)
+
+
+def _state(left, right, p=3, n=12):
+    st = allocate_state_1d(Mesh1D(0.0, 1.0, n, left_bc=left, right_bc=right), build_operators(p))
+    set_initial_condition_1d(st, lambda x: sod_ic(EQ1, x, x0=0.4) * (1.0 + 0.05 * np.sin(7 * x))[..., None])
+    return st
+
+
+def _ends(st):
+    """The end traces in the oracle's summation order (left to right from 0, no fused
+    multiply-add): the same bits as its transmissive ghost states."""
+    ops = st.ops
+    uL0, uRn = np.zeros(3), np.zeros(3)
+    for c in range(3):
+        a = b = 0.0
+        for j in range(ops.n_points):
+            a += ops.ell_L[j] * st.u[j, 0, c]
+            b += ops.ell_R[j] * st.u[j, -1, c]
+        uL0[c], uRn[c] = a, b
+    return uL0, uRn
+
+
+@pytest.mark.parametrize("method", [NullCapturing(), PerssonAVMethod.create(c_av=0.2)], ids=["null", "br0"])
+def test_dirichlet_with_the_end_traces_is_transmissive(method):
+    T = TransmissiveBC
+    st = _state(T(), T())
+    ref = oracle_for_state_1d(st, method, EQ1).residual(st.u)
+    uL0, uRn = _ends(st)
+    st_d = allocate_state_1d(Mesh1D(0.0, 1.0, 12, left_bc=DirichletBC(state=uL0), right_bc=DirichletBC(state=uRn)),
+                             build_operators(3))
+    st_d.u[...] = st.u
+    got = oracle_for_state_1d(st_d, method, EQ1).residual(st_d.u)
+    assert np.array_equal(got, ref)
+    st_d.mesh.left_bc = DirichletBC(state=uL0 * (1.0 + 1e-6))
+    assert not np.array_equal(oracle_for_state_1d(st_d, method, EQ1).residual(st_d.u), ref)
+
+
+def test_dirichlet_changes_only_the_end_elements():
+    T = TransmissiveBC
+    st = _state(T(), T())
+    ref = oracle_for_state_1d(st, NullCapturing(), EQ1).residual(st.u)
+    UD = np.asarray(EQ1.primitives_to_conserved(1.3, 0.2, 1.1))
+    st_d = allocate_state_1d(Mesh1D(0.0, 1.0, 12, left_bc=DirichletBC(state=UD), right_bc=T()), build_operators(3))
+    st_d.u[...] = st.u
+    got = oracle_for_state_1d(st_d, NullCapturing(), EQ1).residual(st_d.u)
+    assert np.array_equal(got[:, 1:, :], ref[:, 1:, :])
+    assert not np.array_equal(got[:, 0, :], ref[:, 0, :])
+
+
+def test_bc_freestream_cases_through_the_oracle():
+    """test_euler.jl:55-63: both cases pass with a deviation below 1e-12."""
+    for run in (run_bc_transmissive_test, run_bc_dirichlet_test):
+        c = run(integrator=oracle_integrator_1d)
+        assert c["pass"] and c["metrics"]["max_deviation_from_uniform"] < 1e-12
+        assert c["metrics"]["n_steps"] > 0

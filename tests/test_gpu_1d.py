@@ -98,3 +98,39 @@ def test_step_dt_and_divergence_1d():
     assert ssp_rk3_step_1d(st, EQ, NullCapturing(), 1e-3) == "diverged"
     assert st.t == 0.0
     np.testing.assert_array_equal(np.isnan(st.u), np.isnan(u_before))
+
+
+@pytest.mark.parametrize("capt", ["null", "br0", "dd"])
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_residual_1d_dirichlet_matches_oracle(p, capt):
+    """Constant Dirichlet ends (Residual.jl:75-79,100-104) on the device against the oracle."""
+    from frforge_amd.mesh import DirichletBC
+    from helpers import oracle_for_state_1d
+    eq = Euler1D(1.4)
+    UL = np.asarray(eq.primitives_to_conserved(1.2, 0.3, 1.4))
+    UR = np.asarray(eq.primitives_to_conserved(0.2, -0.1, 0.15))
+    st = allocate_state_1d(Mesh1D(0.0, 1.0, 23, left_bc=DirichletBC(state=UL), right_bc=DirichletBC(state=UR)),
+                           build_operators(p))
+    set_initial_condition_1d(st, lambda x: sod_ic(eq, x, x0=0.45))
+    m = {"null": NullCapturing(), "br0": PerssonAVMethod.create(c_av=0.2),
+         "dd": PerssonAVMethod.create(c_av=0.2, av_form="element_local_DD")}[capt]
+    du = np.zeros(st.u.shape, order="F")
+    residual_1d(du, st, eq, m)
+    ref = oracle_for_state_1d(st, m, eq).residual(st.u)
+    err = max(np.max(np.abs(du[:, :, c] - ref[:, :, c])) / max(np.max(np.abs(ref[:, :, c])), 1e-300) for c in range(3))
+    print(f"1d dirichlet p={p} {capt}: {err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_bc_freestream_cases_on_the_device():
+    """test_euler.jl:55-63 on the device: the uniform state stays uniform to 1e-12 between
+    transmissive and between Dirichlet ends, with the oracle's step counts."""
+    from frforge_amd.cases1d import run_bc_dirichlet_test, run_bc_transmissive_test
+    from helpers import oracle_integrator_1d
+    for run in (run_bc_transmissive_test, run_bc_dirichlet_test):
+        c = run()
+        o = run(integrator=oracle_integrator_1d)
+        print(f"{c['name']}: dev {c['metrics']['max_deviation_from_uniform']:.3e} "
+              f"steps {c['metrics']['n_steps']} (oracle {o['metrics']['n_steps']})", flush=True)
+        assert c["pass"] and c["metrics"]["max_deviation_from_uniform"] < 1e-12
+        assert c["metrics"]["n_steps"] == o["metrics"]["n_steps"]

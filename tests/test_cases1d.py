@@ -68,3 +68,12 @@ def test_bc_freestream_cases_through_the_oracle():
         c = run(integrator=oracle_integrator_1d)
         assert c["pass"] and c["metrics"]["max_deviation_from_uniform"] < 1e-12
         assert c["metrics"]["n_steps"] > 0
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_euler_conservation_through_the_oracle(p):
+    """test_euler.jl:29-37: periodic mass conserved to < 1e-10 relative, positive, passing."""
+    from frforge_amd.cases1d import run_euler_conservation
+    c = run_euler_conservation(p=p, n_elements=32, t_final=1.0, integrator=oracle_integrator_1d)
+    assert c["pass"] and c["conservation_pass"] and c["positivity_ok"]
+    assert c["conservation_residual"] < 1e-10

@@ -134,3 +134,19 @@ def test_bc_freestream_cases_on_the_device():
               f"steps {c['metrics']['n_steps']} (oracle {o['metrics']['n_steps']})", flush=True)
         assert c["pass"] and c["metrics"]["max_deviation_from_uniform"] < 1e-12
         assert c["metrics"]["n_steps"] == o["metrics"]["n_steps"]
+
+
+def test_m3_euler_suite_on_the_device():
+    """run_m3_euler_suite (Cases.jl:702-745) on the device: every case passes (test_euler.jl:29-63
+    gates), and the order studies' L2 errors equal the oracle's to 1e-10 relative."""
+    from frforge_amd.cases1d import run_m3_euler_suite
+    from helpers import oracle_integrator_1d
+    cases, overall, hard = run_m3_euler_suite()
+    cases_o, overall_o, _ = run_m3_euler_suite(integrator=oracle_integrator_1d)
+    assert overall and overall_o and hard == []
+    for c, o in zip(cases, cases_o):
+        assert c["name"] == o["name"] and c["pass"] == o["pass"]
+        if "l2_errors" in c:
+            print(f"{c['name']}: orders {c['observed_orders']}", flush=True)
+            assert np.allclose(c["l2_errors"], o["l2_errors"], rtol=1e-10, atol=0)
+            assert all(q >= c["formal_order"] - 0.3 for q in c["observed_orders"])

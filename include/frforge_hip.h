@@ -2,8 +2,9 @@
  * frforge_hip.h -- C ABI of the MI355X (gfx950) FRForge 2D FR residual and SSP-RK3 time step.
  *
  * Drop-in boundary for FRForge's Julia API (RcktMan77/FRForge): each entry point below replaces
- * one reference function for SolutionState2D / Euler2D on affine meshes with periodic or
- * transmissive sides.  A Julia `ccall` shim keeps the reference signatures; the mapping is in
+ * one reference function for SolutionState2D / Euler2D (affine or curved meshes; periodic,
+ * transmissive, reflecting, Dirichlet or split-ghost sides; optional solid masks) or for the 1D
+ * Euler1D path.  A Julia `ccall` shim keeps the reference signatures; the mapping is in
  * INTEGRATION.md.
  *
  * Conventions
@@ -134,7 +135,7 @@ int32_t frf_upload_state(frf_ctx* ctx, const double* u, int64_t len, double t);
 int32_t frf_download_state(frf_ctx* ctx, double* u, int64_t len, double* t);
 
 /* residual!(du, state, eq, method) (Residual2D.jl:131-416) on the uploaded state; du is fully
- * overwritten, same layout as u. */
+ * overwritten, same layout as u.  Split ghost sides see the uploaded state.t (:143). */
 int32_t frf_residual(frf_ctx* ctx, double* du, int64_t len);
 
 /* sense! (PerssonAV2D.jl:20-59) and element_viscosities_2d (:89-109) on the uploaded state.
@@ -196,7 +197,7 @@ typedef struct frf1d_problem {
     int32_t p;           /* 1..5 */
     int32_t nel;
     int32_t neq;         /* must be 3 (Euler1D) */
-    int32_t bc_left, bc_right;   /* FRF_BC_PERIODIC (both) or FRF_BC_TRANSMISSIVE */
+    int32_t bc_left, bc_right;   /* FRF_BC_PERIODIC (both), FRF_BC_TRANSMISSIVE or FRF_BC_DIRICHLET */
     int32_t flux;        /* FRF_FLUX_RUSANOV */
     int32_t precision;   /* FRF_FP64 */
     int32_t capturing;   /* FRF_CAPT_* */
@@ -234,9 +235,10 @@ int32_t frf1d_destroy(frf1d_ctx* ctx);
  * No single reference counterpart: it runs what the reference does as a sweep of independent
  * ssp_rk3! / integrate! calls over meshes, degrees and fluxes (the case runners' loops,
  * Cases2D_Riemann.jl:194-255, and the invent/scoring sweeps) as one job.  Instances are contexts
- * set up as above in the FP32-mixed form, without a solid mask.  Instances sharing degree, flux and capturing method
- * advance with one stage launch per group; the step sequence of all groups is captured once in
- * a HIP graph and replayed.  A batch keeps pointers to its contexts: destroy it first. */
+ * set up as above in the FP32-mixed form, without a solid mask or curved metrics.  Instances
+ * sharing degree, flux and capturing method advance with one stage launch per group; the step
+ * sequence of all groups is captured once in a HIP graph and replayed.  A batch keeps pointers
+ * to its contexts: destroy it first. */
 typedef struct frf_batch frf_batch;
 int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch** out);
 

@@ -102,76 +102,6 @@ def time_form(nx, p, precision, capturing, steps, warmup, dt):
     return {"wall_s": wall, "dt": dt, "profile": prof, "u": u, "w": st.ops.w, "J": st.metrics.J}
 
 
-def ensemble_state(inst, precision):
-    """One instance of the seeded ensemble (frforge_amd.cases.ensemble_instances)."""
-    from frforge_amd import (Euler2D, Mesh2D, NullCapturing, PeriodicBC, PerssonAVMethod,
-                             SchemeConfig, TransmissiveBC, allocate_state, build_operators,
-                             set_initial_condition)
-    from frforge_amd.cases import riemann2d_ic, seeded_quadrant_states, vortex_ic
-    eq = Euler2D(1.4)
-    ops = build_operators(inst["p"])
-    n = inst["n"]
-    if inst["kind"] == "vortex":   # periodic [-5,5]^2, vortex at the origin, no capturing
-        B = PeriodicBC
-        mesh = Mesh2D(-5.0, 5.0, -5.0, 5.0, n, n, left_bc=B(), right_bc=B(), bottom_bc=B(), top_bc=B())
-        st = allocate_state(mesh, ops, scheme=SchemeConfig(flux=inst["flux"]), precision=precision)
-        set_initial_condition(st, vortex_ic(eq, x0=0.0, y0=0.0))
-        method = NullCapturing()
-    else:                          # transmissive [0,1]^2, seeded quadrants, Persson + BR0
-        B = TransmissiveBC
-        mesh = Mesh2D(0.0, 1.0, 0.0, 1.0, n, n, left_bc=B(), right_bc=B(), bottom_bc=B(), top_bc=B())
-        st = allocate_state(mesh, ops, scheme=SchemeConfig(flux=inst["flux"]), precision=precision)
-        cfg = seeded_quadrant_states(inst["seed"])
-        set_initial_condition(st, lambda x, y: riemann2d_ic(eq, x, y, config=cfg))
-        method = PerssonAVMethod.create(c_av=0.1)
-    return eq, st, method
-
-
-def run_ensemble(steps, seed=SEED, count=64):
-    """The batched form on the seeded ensemble: `count` instances of uneven mesh (64^2..512^2),
-    degree (2..5) and flux stepped together (frf_batch_run, autotuned tiles), `steps` fixed-dt
-    steps each (CFL 0.08 of the instance's initial state), against the same steps in FP64."""
-    from frforge_amd import compute_dt, ssp_rk3, ssp_rk3_batch
-    from frforge_amd.cases import ensemble_instances
-    insts = ensemble_instances(seed, count)
-    built = [ensemble_state(i, "fp32_mixed") for i in insts]
-    eq = built[0][0]
-    states = [b[1] for b in built]
-    methods = [b[2] for b in built]
-    dts = [compute_dt(s, eq, cfl=CFL) for s in states]
-    u_init = [s.u.copy(order="F") for s in states]
-    res, ms = ssp_rk3_batch(states, eq, methods, math.inf, dt=dts, max_steps=steps, tune=True)
-    dof = [s.u.size for s in states]
-    errs, agree = [], 0
-    for inst, st, m, dt, u0, r in zip(insts, states, methods, dts, u_init, res):
-        _, ref, _ = ensemble_state(inst, "fp64")
-        ref.u[...] = u0
-        r64 = ssp_rk3(ref, eq, m, math.inf, dt=dt, max_steps=steps)
-        agree += (r64["status"], r64["n_steps"]) == (r["status"], r["n_steps"])
-        if r["status"] == r64["status"] == "max_steps":
-            w = st.ops.w
-            W = w[:, None, None] * w[None, :, None] * np.abs(ref.metrics.J)
-            d = np.sum(W * np.abs(st.u[..., 0] - ref.u[..., 0]))
-            errs.append(float(d / np.sum(W * np.abs(ref.u[..., 0]))))
-        for s in (st, ref):
-            if s.device is not None:
-                s.device.close()
-                s.device = None
-    # work actually done: an instance that diverges stops at its failing step
-    total = 3.0 * sum(n * r["n_steps"] for n, r in zip(dof, res))
-    return {"instances": count, "seed": seed, "steps": steps, "dof": int(sum(dof)),
-            "completed": sum(r["status"] == "max_steps" for r in res),
-            "diverged": sum(r["status"] == "diverged" for r in res),
-            "status_and_steps_match_fp64": agree,
-            "ms": ms, "dof_stage_per_s": total / (ms * 1e-3),
-            "hbm_GBs_algorithmic": total * (64.0 / 3.0) / (ms * 1e-3) / 1e9,
-            "density_rel_l1_vs_fp64": {"max": max(errs) if errs else None,
-                                       "median": float(np.median(errs)) if errs else None,
-                                       "instances": len(errs)},
-            "form": "fp32_mixed, tiles autotuned per (degree, flux, capturing) group",
-            "mix": {"meshes": sorted({i["n"] for i in insts}), "degrees": sorted({i["p"] for i in insts})}}
-
-
 def run_cpu_baseline(nx, p, dt, capturing="persson_br0"):
     from oracle import oracle as O
     eq, st, method, _ = build_state(nx, p, "fp64", capturing)
@@ -320,7 +250,8 @@ def main():
 
     ens = None
     if not a.no_ensemble:
-        ens = run_ensemble(a.ensemble_steps)
+        from frforge_amd.ensemble import run_ensemble
+        ens = run_ensemble(a.ensemble_steps, seed=SEED, cfl=CFL)
         print(f"# ensemble: {ens['instances']} instances, {ens['dof'] / 1e6:.0f} M DOF, "
               f"{ens['ms'] / ens['steps']:.3f} ms/step, {ens['dof_stage_per_s']:.3e} DOF-stage/s",
               file=sys.stderr, flush=True)

@@ -6,6 +6,7 @@
 // stages 2 and 3 read u^(k) and u^n and write one state.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -852,6 +853,7 @@ struct frf_batch {
     std::vector<frf_ctx*> inst;
     std::vector<Group> groups;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> side;                      // fork streams of the captured graph
     DevCtl** dCtls = nullptr;
     int* dPending = nullptr;
     int* hPending = nullptr;                            // pinned
@@ -862,6 +864,7 @@ struct frf_batch {
 };
 
 static constexpr int kBatchGraphSteps = 8;   // SSP-RK3 steps per graph replay
+static constexpr size_t kBatchSideStreams = 8;   // parallel branches of a captured step
 
 static int32_t bfail(frf_batch* b, int32_t code, const char* fmt, ...) {
     char buf[512];
@@ -892,6 +895,7 @@ extern "C" int32_t frf_batch_destroy(frf_batch* b) {
     if (b->dCtls) (void)hipFree(b->dCtls);
     if (b->dPending) (void)hipFree(b->dPending);
     if (b->hPending) (void)hipHostFree(b->hPending);
+    for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return FRF_OK;
@@ -956,6 +960,12 @@ extern "C" int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch**
         return FRF_E_HIP;
     };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+    const size_t nside = std::min<size_t>(b->groups.size(), kBatchSideStreams);
+    for (size_t k = 0; nside > 1 && k < nside; ++k) {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return cleanup("stream");
+        b->side.push_back(s);
+    }
     if (hipMalloc(&b->dCtls, n * sizeof(DevCtl*)) != hipSuccess ||
         hipMalloc(&b->dPending, sizeof(int)) != hipSuccess ||
         hipHostMalloc(&b->hPending, sizeof(int), hipHostMallocDefault) != hipSuccess)
@@ -1003,10 +1013,11 @@ static int32_t batch_upload_group(frf_batch* b, frf_batch::Group& g, const std::
     return FRF_OK;
 }
 
-static int32_t batch_enqueue_stage(frf_batch* b, frf_batch::Group& g, int stage) {
+static int32_t batch_enqueue_stage(frf_batch* b, frf_batch::Group& g, int stage, hipStream_t s = nullptr) {
     const frf_ctx* c0 = b->inst[g.members[0]];
     BHIPCHK(b, frf::launch_stage_mixed_group(g.np, g.variant, g.flux, g.capt, g.dP[stage - 1], g.dStart,
-                                             (int)g.members.size(), g.start.back(), c0->ops, b->stream));
+                                             (int)g.members.size(), g.start.back(), c0->ops,
+                                             s ? s : b->stream));
     return FRF_OK;
 }
 
@@ -1070,7 +1081,9 @@ extern "C" int32_t frf_batch_tune(frf_batch* b, int32_t repeats) {
     return r;
 }
 
-// Capture kBatchGraphSteps steps of every instance (dt, three stages per group, finish).
+// Capture kBatchGraphSteps steps of every instance: dt, then the three stages of each group as a
+// chain of its own (groups hold disjoint instances, so a group's stage depends only on its own
+// previous stage), forked over the side streams and joined before finish.
 static int32_t batch_capture(frf_batch* b) {
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
@@ -1081,18 +1094,41 @@ static int32_t batch_capture(frf_batch* b) {
         b->graph = nullptr;
     }
     const int n = (int)b->inst.size();
-    BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const size_t ns = b->side.size(), ng = b->groups.size();
+    std::vector<hipEvent_t> ev(ns ? ng + 1 : 0, nullptr);   // fork, then one join per group
+    for (auto& e : ev) BHIPCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auto destroy_events = [&]() {
+        for (auto e : ev) if (e) (void)hipEventDestroy(e);
+    };
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        destroy_events();
+        return bfail(b, FRF_E_HIP, "hipStreamBeginCapture failed");
+    }
     int32_t r = FRF_OK;
     for (int s = 0; s < kBatchGraphSteps && r == FRF_OK; ++s) {
         if (frf::launch_dt_group(b->dCtls, n, b->stream) != hipSuccess) r = bfail(b, FRF_E_HIP, "dt launch");
-        for (int stage = 1; stage <= 3 && r == FRF_OK; ++stage)
-            for (auto& g : b->groups)
-                if (r == FRF_OK) r = batch_enqueue_stage(b, g, stage);
+        if (r == FRF_OK && ns) {
+            if (hipEventRecord(ev[0], b->stream) != hipSuccess) r = bfail(b, FRF_E_HIP, "fork event");
+            for (size_t gi = 0; gi < ng && r == FRF_OK; ++gi) {
+                hipStream_t sg = b->side[gi % ns];
+                if (hipStreamWaitEvent(sg, ev[0], 0) != hipSuccess) r = bfail(b, FRF_E_HIP, "fork wait");
+                for (int stage = 1; stage <= 3 && r == FRF_OK; ++stage)
+                    r = batch_enqueue_stage(b, b->groups[gi], stage, sg);
+                if (r == FRF_OK && hipEventRecord(ev[1 + gi], sg) != hipSuccess) r = bfail(b, FRF_E_HIP, "join event");
+            }
+            for (size_t gi = 0; gi < ng && r == FRF_OK; ++gi)
+                if (hipStreamWaitEvent(b->stream, ev[1 + gi], 0) != hipSuccess) r = bfail(b, FRF_E_HIP, "join wait");
+        } else {
+            for (int stage = 1; stage <= 3 && r == FRF_OK; ++stage)
+                for (auto& g : b->groups)
+                    if (r == FRF_OK) r = batch_enqueue_stage(b, g, stage);
+        }
         if (r == FRF_OK && frf::launch_finish_group(b->dCtls, n, b->stream) != hipSuccess)
             r = bfail(b, FRF_E_HIP, "finish launch");
     }
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    destroy_events();
     if (r) {
         if (graph) (void)hipGraphDestroy(graph);
         return r;

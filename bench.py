@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched 64-instance ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=16)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ensemble leg and print its summary (for rocprof)")
     ap.add_argument("--forms", default="fp32_mixed:persson_br0,fp64:persson_br0,fp32_mixed:null,fp64:null,"
                                        "fp32_mixed:learned")
     ap.add_argument("--profile-only", default="", help="time only these form:capturing pairs (for rocprof)")
@@ -191,6 +193,10 @@ def main():
     if device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if a.ensemble_only:
+        from frforge_amd.ensemble import run_ensemble
+        print(json.dumps(run_ensemble(a.ensemble_steps, seed=SEED, cfl=CFL)), flush=True)
+        return 0
     nx, p = a.nx, a.p
     nel = nx * nx
     dof = nel * (p + 1) ** 2 * 4

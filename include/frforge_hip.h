@@ -237,7 +237,8 @@ int32_t frf1d_destroy(frf1d_ctx* ctx);
  * Cases2D_Riemann.jl:194-255, and the invent/scoring sweeps) as one job.  Instances are contexts
  * set up as above in the FP32-mixed form, without a solid mask or curved metrics.  Instances
  * sharing degree, flux and capturing method advance with one stage launch per group; the step
- * sequence of all groups is captured once in a HIP graph and replayed.  A batch keeps pointers
+ * (dt, each group's stage chain as a parallel branch, finish) is captured once in a HIP graph
+ * and replayed.  A batch keeps pointers
  * to its contexts: destroy it first. */
 typedef struct frf_batch frf_batch;
 int32_t frf_batch_create(frf_ctx* const* ctxs, int32_t n, frf_batch** out);
